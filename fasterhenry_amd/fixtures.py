@@ -1,0 +1,165 @@
+"""Synthetic geometries the benchmarks and tests run on.
+
+* ``meander`` -- fasterhenry/benches/support.rs:104-127 (the scaling fixture:
+  ``meander(49)`` at 2 x 2 is the 9 800-filament case, ``meander(24)`` the
+  2 400-filament one).
+* ``serpentine`` -- fasterhenry/benches/support.rs:64-83.
+* ``busbar_pair`` -- two parallel copper bars shorted at the far end, one port.
+* ``coupled_structure`` -- fasterhenry/tests/common/mod.rs:44-100.
+* ``spiral_system`` -- fasterhenry/tests/common/mod.rs:102-157, the 2-turn
+  square spiral whose published result is L(1 kHz) = 0.637460 nH and
+  R(DC) = 2.844828 ohm (docs/validation.md:107-112).
+"""
+from __future__ import annotations
+
+from typing import List, Tuple
+
+import numpy as np
+
+from .model import (Discretization, Geometry, Node, Port, SegmentDef, Subdivision)
+
+COPPER = 5.8e7
+
+#: benches/support.rs:43-44
+SUBDIVISION_NW = 2
+SUBDIVISION_NH = 2
+#: benches/support.rs:89-96
+PITCH = 100e-6
+WIDTH = 20e-6
+THICKNESS = 10e-6
+
+
+def meander(bars: int) -> Tuple[Geometry, Port]:
+    """Square meander of ``bars`` runs, each ``bars`` pitch-long segments."""
+    g = Geometry()
+    previous = g.add_node(Node(0.0, 0.0, 0.0))
+    first = previous
+
+    def add(x: float, y: float) -> None:
+        nonlocal previous
+        node = g.add_node(Node(x, y, 0.0))
+        g.add_segment(SegmentDef(previous, node, WIDTH, THICKNESS, COPPER))
+        previous = node
+
+    for bar in range(bars):
+        y = bar * PITCH
+        for step in range(1, bars + 1):
+            along = step if bar % 2 == 0 else bars - step
+            add(along * PITCH, y)
+        x = (bars if bar % 2 == 0 else 0) * PITCH
+        add(x, y + PITCH)
+    return g, Port(first, previous)
+
+
+def meander_filament_count(bars: int) -> int:
+    """benches/scaling.rs:99-101"""
+    return bars * (bars + 1) * SUBDIVISION_NW * SUBDIVISION_NH
+
+
+def meander_problem(bars: int):
+    g, p = meander(bars)
+    return g, [p], Discretization.uniform(SUBDIVISION_NW, SUBDIVISION_NH)
+
+
+def serpentine(bars: int) -> Tuple[Geometry, Port]:
+    """benches/support.rs:64-83"""
+    length, pitch, width, thickness = 2e-3, 60e-6, 20e-6, 10e-6
+    g = Geometry()
+    previous = g.add_node(Node(0.0, 0.0, 0.0))
+    first = previous
+    for bar in range(bars):
+        y = bar * pitch
+        x_far = length if bar % 2 == 0 else 0.0
+        for (x, yy) in ((x_far, y), (x_far, y + pitch)):
+            node = g.add_node(Node(x, yy, 0.0))
+            g.add_segment(SegmentDef(previous, node, width, thickness, COPPER))
+            previous = node
+    return g, Port(first, previous)
+
+
+def busbar_pair(segments_per_bar: int = 10, nw: int = 8, nh: int = 4):
+    """Two parallel copper bars, 10 mm x 1.0 mm x 0.1 mm, 0.5 mm edge gap,
+    shorted at the far end by a single-filament strap; one port across the
+    near ends.  640 bar filaments + 1 strap at the defaults."""
+    length, width, height, gap = 10e-3, 1.0e-3, 0.1e-3, 0.5e-3
+    g = Geometry()
+    ya, yb = 0.0, width + gap
+    a_nodes = [g.add_node(Node(length * k / segments_per_bar, ya, 0.0))
+               for k in range(segments_per_bar + 1)]
+    b_nodes = [g.add_node(Node(length * k / segments_per_bar, yb, 0.0))
+               for k in range(segments_per_bar + 1)]
+    subs: List[Subdivision] = []
+    for k in range(segments_per_bar):
+        g.add_segment(SegmentDef(a_nodes[k], a_nodes[k + 1], width, height, COPPER))
+        subs.append(Subdivision(nw, nh))
+    for k in range(segments_per_bar):
+        g.add_segment(SegmentDef(b_nodes[k + 1], b_nodes[k], width, height, COPPER))
+        subs.append(Subdivision(nw, nh))
+    g.add_segment(SegmentDef(a_nodes[-1], b_nodes[-1], width, height, COPPER))
+    subs.append(Subdivision.SINGLE)
+    return g, [Port(a_nodes[0], b_nodes[0])], Discretization.per_segment(subs)
+
+
+def coupled_structure():
+    """tests/common/mod.rs:44-100: 11 nodes, 8 segments, 41 filaments, a ring."""
+    points = [
+        [0.0, 0.0, 0.0], [4e-3, 0.0, 0.0], [4e-3, 2e-3, 0.0],
+        [0.5e-3, 0.0, 0.3e-3], [3.5e-3, 0.0, 0.3e-3],
+        [0.0, -1e-3, 0.0], [3e-3, -1e-3, 0.0], [3e-3, -3e-3, 0.0], [0.0, -3e-3, 0.0],
+        [1e-3, 1e-3, 0.0], [3e-3, 1e-3, 0.0],
+    ]
+    segments = [
+        (0, 1, 0.4e-3, 0.1e-3), (1, 2, 0.4e-3, 0.1e-3), (3, 4, 0.2e-3, 0.05e-3),
+        (5, 6, 0.2e-3, 0.1e-3), (6, 7, 0.2e-3, 0.1e-3), (7, 8, 0.2e-3, 0.1e-3),
+        (8, 5, 0.2e-3, 0.1e-3), (9, 10, 0.2e-3, 0.1e-3),
+    ]
+    g = Geometry()
+    for p in points:
+        g.add_node(Node(*p))
+    for a, b, w, h in segments:
+        g.add_segment(SegmentDef(a, b, w, h, COPPER))
+    ports = [Port(0, 2, "A"), Port(4, 3, "B"), Port(5, 7, "ring diagonal")]
+    subs = [Subdivision(2, 2)] * len(segments)
+    subs[0] = Subdivision(4, 3)
+    subs[1] = Subdivision(3, 3)
+    subs[2] = Subdivision(3, 1)
+    subs[5] = Subdivision.SINGLE
+    return g, ports, Discretization.per_segment(subs)
+
+
+def square_spiral_segments(turns: int, start_len_um: float, pitch_um: float):
+    """tests/common/mod.rs:123-142"""
+    directions = [(1.0, 0.0), (0.0, 1.0), (-1.0, 0.0), (0.0, -1.0)]
+    position = (0.0, 0.0)
+    length = start_len_um
+    out = []
+    for k in range(turns * 4):
+        d = directions[k % 4]
+        end = (position[0] + d[0] * length, position[1] + d[1] * length)
+        out.append((position, end))
+        position = end
+        if k % 2 == 1:
+            length += pitch_um
+    return out
+
+
+def spiral_system():
+    """tests/common/mod.rs:144-157: 2 turns, 60 um start, 15 um pitch, 2x2 um
+    copper, 4 x 4 filaments per segment (128 filaments)."""
+    um, w_um, t_um = 1e-6, 2.0, 2.0
+    segs = square_spiral_segments(2, 60.0, 15.0)
+    nodes = [Node(0.0, 0.0, t_um / 2.0 * um)]
+    for (_, end) in segs:
+        nodes.append(Node(end[0] * um, end[1] * um, t_um / 2.0 * um))
+    defs = [SegmentDef(i, i + 1, w_um * um, t_um * um, COPPER) for i in range(len(segs))]
+    g = Geometry.from_parts(nodes, defs)
+    return g, [Port(0, len(segs), "spiral")], Discretization.per_segment(
+        [Subdivision(4, 4)] * len(segs))
+
+
+def log_frequencies(f0: float, f1: float, count: int) -> np.ndarray:
+    """f_k = f0 (f1/f0)^(k/(count-1))."""
+    if count == 1:
+        return np.array([f0])
+    k = np.arange(count, dtype=np.float64)
+    return f0 * (f1 / f0) ** (k / (count - 1))

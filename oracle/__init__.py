@@ -1,0 +1,320 @@
+"""ctypes binding of the oracle (oracle/fh_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the CPU FP64 restatement of the reference's hot
+path that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+use as the checker.  The product (fasterhenry_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfh_oracle.so")
+
+METHODS = {0: "orthogonal", 1: "bar_closed_form", 2: "aligned_quadrature",
+           3: "point_quadrature", 4: "sampled_filaments"}
+
+FHO_OK = 0
+FHO_E_KERNEL = 4
+FHO_E_MESH = 5
+FHO_E_INVALID_FREQUENCY = 7
+FHO_E_SINGULAR = 8
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status: int, detail: "Detail"):
+        super().__init__(f"oracle status {status}: code {detail.code} "
+                         f"index ({detail.index0}, {detail.index1}) value {detail.value}")
+        self.status = status
+        self.detail = detail
+
+
+class Detail(C.Structure):
+    _fields_ = [("index0", C.c_int64), ("index1", C.c_int64), ("value", C.c_double),
+                ("code", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"{_LIB_PATH} is missing: run `make -C oracle` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        vp = C.c_void_p
+        sz = C.c_size_t
+        L.fho_set_math.argtypes = [C.c_int]
+        L.fho_log.restype = C.c_double
+        L.fho_log.argtypes = [C.c_double]
+        L.fho_atan.restype = C.c_double
+        L.fho_atan.argtypes = [C.c_double]
+        L.fho_hypot.restype = C.c_double
+        L.fho_hypot.argtypes = [C.c_double, C.c_double]
+        L.fho_last_error.argtypes = [C.POINTER(Detail)]
+        L.fho_gauss_rule.argtypes = [C.c_int, dp, dp]
+        L.fho_order_for.restype = C.c_int
+        L.fho_order_for.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double]
+        L.fho_bar_integral.restype = C.c_double
+        L.fho_bar_integral.argtypes = [dp, dp, dp, dp]
+        L.fho_corner_differences.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+        L.fho_lines_parallel.restype = C.c_int
+        L.fho_lines_parallel.argtypes = [C.c_double] * 4 + [dp]
+        L.fho_lines_skew.restype = C.c_int
+        L.fho_lines_skew.argtypes = [dp, dp, dp, dp, C.c_double, dp]
+        L.fho_aligned_integral.restype = C.c_double
+        L.fho_aligned_integral.argtypes = [dp]
+        L.fho_aligned_dispatch.restype = C.c_double
+        L.fho_aligned_dispatch.argtypes = [dp, C.POINTER(C.c_int)]
+        L.fho_discretize_geometry.restype = C.c_int
+        L.fho_discretize_geometry.argtypes = [vp, sz, vp, sz, vp, vp, sz, C.POINTER(sz)]
+        L.fho_mutual.restype = C.c_int
+        L.fho_mutual.argtypes = [vp, vp, dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.fho_self_inductance.restype = C.c_double
+        L.fho_self_inductance.argtypes = [vp]
+        L.fho_fill.restype = C.c_int
+        L.fho_fill.argtypes = [vp, sz, vp, C.c_int]
+        L.fho_fill_pairs.restype = C.c_int
+        L.fho_fill_pairs.argtypes = [vp, sz, vp, vp, sz, vp, vp, C.c_int]
+        L.fho_mesh_build.restype = C.c_int
+        L.fho_mesh_build.argtypes = [sz, vp, sz, vp, vp, sz, vp, vp, vp, C.POINTER(sz),
+                                     C.POINTER(sz), C.POINTER(sz)]
+        L.fho_congruence.restype = C.c_int
+        L.fho_congruence.argtypes = [sz, vp, vp, vp, sz, vp, vp, vp, C.c_int]
+        L.fho_lu_solve_complex.restype = C.c_int
+        L.fho_lu_solve_complex.argtypes = [vp, sz, vp, sz, C.c_int]
+        L.fho_lu_solve_real.restype = C.c_int
+        L.fho_lu_solve_real.argtypes = [vp, sz, vp, sz, C.c_int]
+        L.fho_system_create.restype = C.c_int
+        L.fho_system_create.argtypes = [vp, sz, vp, sz, vp, vp, sz, C.c_int, C.POINTER(vp)]
+        L.fho_system_destroy.argtypes = [vp]
+        L.fho_system_counts.argtypes = [vp, C.POINTER(sz)]
+        L.fho_system_blocks.restype = C.c_int
+        L.fho_system_blocks.argtypes = [vp] * 7
+        L.fho_system_inductance.restype = vp
+        L.fho_system_inductance.argtypes = [vp]
+        L.fho_system_filaments.restype = vp
+        L.fho_system_filaments.argtypes = [vp]
+        L.fho_system_sweep.restype = C.c_int
+        L.fho_system_sweep.argtypes = [vp, vp, sz, vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status: int) -> None:
+    if status != FHO_OK:
+        d = Detail()
+        lib().fho_last_error(C.byref(d))
+        raise OracleError(status, d)
+
+
+def set_portable_math(portable: bool) -> None:
+    lib().fho_set_math(1 if portable else 0)
+
+
+def log(x: float) -> float:
+    return lib().fho_log(x)
+
+
+def atan(x: float) -> float:
+    return lib().fho_atan(x)
+
+
+def gauss_rule(n: int) -> Tuple[np.ndarray, np.ndarray]:
+    nodes = np.zeros(max(n, 1))
+    weights = np.zeros(max(n, 1))
+    dp = C.POINTER(C.c_double)
+    lib().fho_gauss_rule(n, nodes.ctypes.data_as(dp), weights.ctypes.data_as(dp))
+    return nodes, weights
+
+
+def order_for(tolerance: float, max_order: int, half_extent: float, distance: float):
+    r = lib().fho_order_for(tolerance, max_order, half_extent, distance)
+    return None if r == 0 else r
+
+
+def corner_differences(e1: float, e2: float, offset: float) -> np.ndarray:
+    out = np.zeros(4)
+    lib().fho_corner_differences(e1, e2, offset, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def bar_integral(qx, qy, qz) -> Tuple[float, float]:
+    dp = C.POINTER(C.c_double)
+    a, b, c = (np.ascontiguousarray(q, dtype=np.float64) for q in (qx, qy, qz))
+    err = C.c_double()
+    v = lib().fho_bar_integral(a.ctypes.data_as(dp), b.ctypes.data_as(dp), c.ctypes.data_as(dp),
+                               C.byref(err))
+    return v, err.value
+
+
+def lines_parallel(l1: float, l2: float, offset: float, rho: float):
+    out = C.c_double()
+    ok = lib().fho_lines_parallel(l1, l2, offset, rho, C.byref(out))
+    return out.value if ok else None
+
+
+def lines_skew(a0, a1, b0, b1, sin_min: float = 0.0):
+    dp = C.POINTER(C.c_double)
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (a0, a1, b0, b1)]
+    out = C.c_double()
+    ok = lib().fho_lines_skew(*(x.ctypes.data_as(dp) for x in arrs), sin_min, C.byref(out))
+    return out.value if ok else None
+
+
+def aligned_integral(bars) -> float:
+    b = np.ascontiguousarray(bars, dtype=np.float64)
+    return lib().fho_aligned_integral(b.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+def aligned_dispatch(bars) -> Tuple[float, str]:
+    b = np.ascontiguousarray(bars, dtype=np.float64)
+    m = C.c_int()
+    v = lib().fho_aligned_dispatch(b.ctypes.data_as(C.POINTER(C.c_double)), C.byref(m))
+    return v, METHODS[m.value]
+
+
+def discretize(packed) -> np.ndarray:
+    """Filaments (FILAMENT_DTYPE) of a fasterhenry_amd.model.PackedProblem."""
+    from fasterhenry_amd.model import FILAMENT_DTYPE
+
+    n = C.c_size_t()
+    L = lib()
+    _check(L.fho_discretize_geometry(_ptr(packed.nodes), len(packed.nodes), _ptr(packed.segments),
+                                     len(packed.segments), _ptr(packed.grids), None, 0,
+                                     C.byref(n)))
+    out = np.zeros(n.value, dtype=FILAMENT_DTYPE)
+    _check(L.fho_discretize_geometry(_ptr(packed.nodes), len(packed.nodes), _ptr(packed.segments),
+                                     len(packed.segments), _ptr(packed.grids), _ptr(out), n.value,
+                                     C.byref(n)))
+    return out
+
+
+def mutual(fa: np.ndarray, fb: np.ndarray) -> Tuple[float, str, bool]:
+    """mutual_inductance_detailed of two filaments (FILAMENT_DTYPE scalars)."""
+    a = np.array(fa).reshape(1)
+    b = np.array(fb).reshape(1)
+    v = C.c_double()
+    m = C.c_int()
+    r = C.c_int()
+    _check(lib().fho_mutual(_ptr(a), _ptr(b), C.byref(v), C.byref(m), C.byref(r)))
+    return v.value, METHODS[m.value], bool(r.value)
+
+
+def self_inductance(f: np.ndarray) -> float:
+    a = np.array(f).reshape(1)
+    return lib().fho_self_inductance(_ptr(a))
+
+
+def fill(filaments: np.ndarray, threads: int = 0) -> np.ndarray:
+    n = len(filaments)
+    L = np.zeros((n, n))
+    _check(lib().fho_fill(_ptr(filaments), n, _ptr(L), threads))
+    return L
+
+
+def fill_pairs(filaments: np.ndarray, rows, cols, threads: int = 0):
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.zeros(len(rows))
+    methods = np.zeros(len(rows), dtype=np.int32)
+    _check(lib().fho_fill_pairs(_ptr(filaments), len(filaments), _ptr(rows), _ptr(cols),
+                                len(rows), _ptr(out), _ptr(methods), threads))
+    return out, methods
+
+
+def mesh_build(packed):
+    """CSR loop basis: (row_ptr, branch, sign, internal)."""
+    L = lib()
+    per = (packed.grids["nw"].astype(np.uint64) * packed.grids["nh"]).astype(np.uint64)
+    per = np.ascontiguousarray(per, dtype=np.uintp)
+    loops, internal, nnz = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    _check(L.fho_mesh_build(len(packed.nodes), _ptr(packed.segments), len(packed.segments),
+                            _ptr(per), _ptr(packed.ports), len(packed.ports), None, None, None,
+                            C.byref(loops), C.byref(internal), C.byref(nnz)))
+    row_ptr = np.zeros(loops.value + 1, dtype=np.int64)
+    branch = np.zeros(max(nnz.value, 1), dtype=np.int64)
+    sign = np.zeros(max(nnz.value, 1))
+    _check(L.fho_mesh_build(len(packed.nodes), _ptr(packed.segments), len(packed.segments),
+                            _ptr(per), _ptr(packed.ports), len(packed.ports), _ptr(row_ptr),
+                            _ptr(branch), _ptr(sign), C.byref(loops), C.byref(internal),
+                            C.byref(nnz)))
+    return row_ptr, branch[:nnz.value], sign[:nnz.value], internal.value
+
+
+def lu_solve(a: np.ndarray, b: np.ndarray, threads: int = 0) -> np.ndarray:
+    """dense.rs lu_solve: x = a^-1 b (complex or real)."""
+    complex_ = np.iscomplexobj(a) or np.iscomplexobj(b)
+    dt = np.complex128 if complex_ else np.float64
+    A = np.asfortranarray(a, dtype=dt).copy(order="F")
+    B = np.asfortranarray(b.reshape(len(b), -1), dtype=dt).copy(order="F")
+    f = lib().fho_lu_solve_complex if complex_ else lib().fho_lu_solve_real
+    _check(f(_ptr(A), A.shape[0], _ptr(B), B.shape[1], threads))
+    return B.reshape(b.shape)
+
+
+class System:
+    """An assembled oracle mesh system (solve.rs MeshSystem)."""
+
+    def __init__(self, packed, threads: int = 0):
+        self._h = C.c_void_p()
+        st = lib().fho_system_create(_ptr(packed.nodes), len(packed.nodes), _ptr(packed.segments),
+                                     len(packed.segments), _ptr(packed.grids), _ptr(packed.ports),
+                                     len(packed.ports), threads, C.byref(self._h))
+        _check(st)
+        self.threads = threads
+        c = (C.c_size_t * 6)()
+        lib().fho_system_counts(self._h, c)
+        (self.nodes, self.segments, self.filaments, self.meshes, self.internal,
+         self.ports) = list(c)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().fho_system_destroy(self._h)
+            self._h = None
+
+    @property
+    def counts(self) -> dict:
+        return dict(nodes=self.nodes, segments=self.segments, filaments=self.filaments,
+                    meshes=self.meshes, internal_meshes=self.internal, ports=self.ports)
+
+    def blocks(self):
+        m, p = self.internal, self.ports
+        out = [np.zeros((m, m)), np.zeros((m, m)), np.zeros((m, p)), np.zeros((m, p)),
+               np.zeros((p, p)), np.zeros((p, p))]
+        lib().fho_system_blocks(self._h, *(_ptr(x) for x in out))
+        r_ee, l_ee, r_ep, l_ep, r_pp, l_pp = out
+        return dict(r_ee=r_ee, l_ee=l_ee, r_ep=r_ep, l_ep=l_ep, r_pp=r_pp, l_pp=l_pp)
+
+    def inductance(self) -> np.ndarray:
+        n = self.filaments
+        ptr = lib().fho_system_inductance(self._h)
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(n, n)).copy()
+
+    def filament_array(self) -> np.ndarray:
+        from fasterhenry_amd.model import FILAMENT_DTYPE
+
+        ptr = lib().fho_system_filaments(self._h)
+        raw = (C.c_char * (self.filaments * FILAMENT_DTYPE.itemsize)).from_address(ptr)
+        return np.frombuffer(bytes(raw), dtype=FILAMENT_DTYPE).copy()
+
+    def sweep(self, freqs) -> np.ndarray:
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        z = np.zeros((len(f), self.ports, self.ports), dtype=np.complex128)
+        _check(lib().fho_system_sweep(self._h, _ptr(f), len(f), _ptr(z), self.threads))
+        return z
+
+
+def solve(packed, freqs, threads: int = 0) -> np.ndarray:
+    return System(packed, threads).sweep(freqs)

@@ -1,0 +1,169 @@
+/*
+ * fh_oracle.h -- CPU FP64 restatement of fasterhenry's extraction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the GPU
+ * product (fasterhenry_amd, libfasterhenry_gpu.so) is compared against and
+ * the CPU baseline bench.py times beside it.  Nothing in the product links,
+ * loads or calls it; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg do.
+ *
+ * Every function follows the reference algorithm it names (paths relative to
+ * the reference repository root).  Plain data in, caller-owned buffers out.
+ */
+#ifndef FH_ORACLE_H
+#define FH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; the same numbering as the product's fh_status. */
+enum {
+  FHO_OK = 0,
+  FHO_E_INVALID_ARGUMENT = 1,
+  FHO_E_GEOMETRY = 2,
+  FHO_E_DISCRETIZE = 3,
+  FHO_E_KERNEL = 4,
+  FHO_E_MESH = 5,
+  FHO_E_INVALID_GRADING = 6,
+  FHO_E_INVALID_FREQUENCY = 7,
+  FHO_E_SINGULAR = 8,
+  FHO_E_NO_MEMORY = 11
+};
+
+/* Methods of inductance/mod.rs:130-152. */
+enum {
+  FHO_METHOD_ORTHOGONAL = 0,
+  FHO_METHOD_BAR_CLOSED_FORM = 1,
+  FHO_METHOD_ALIGNED_QUADRATURE = 2,
+  FHO_METHOD_POINT_QUADRATURE = 3,
+  FHO_METHOD_SAMPLED_FILAMENTS = 4
+};
+
+typedef struct {
+  uint32_t a, b;            /* node indices */
+  double width, height, sigma;
+  double width_dir[3];      /* used when has_width_dir != 0 */
+  uint32_t has_width_dir;
+  uint32_t pad_;
+} fho_segment;
+
+typedef struct {
+  uint32_t nw, nh;
+  double width_ratio, height_ratio;
+  uint32_t graded;          /* 0: discretize (filament.rs:320); 1: discretize_graded_per_axis (:478) */
+  uint32_t pad_;
+} fho_grid;
+
+typedef struct {
+  uint32_t positive, negative;
+} fho_port;
+
+typedef struct {
+  double start[3], end[3];
+  double length;
+  double dir[3], wdir[3], hdir[3];
+  double width, height, sigma;
+} fho_filament;
+
+/* Error detail of the last failing call on this thread. */
+typedef struct {
+  int64_t index0, index1;   /* segment / pair row / frequency index ... */
+  double value;
+  int code;                 /* sub-code (e.g. which MeshError) */
+} fho_error;
+
+/* 0: transcendental functions from the host libm (as the reference's Rust
+ * std does); 1: the portable IEEE-basic-operation implementations that the
+ * GPU product also uses, so that both give identical bits. */
+void fho_set_math(int portable);
+int fho_get_math(void);
+double fho_log(double x);
+double fho_atan(double x);
+double fho_hypot(double x, double y);
+
+void fho_last_error(fho_error *out);
+
+/* gauss.rs:25-70 -- n-point Gauss-Legendre rule on [-1, 1]. */
+void fho_gauss_rule(int n, double *nodes, double *weights);
+/* gauss.rs:100-137 -- first order whose threshold delta reaches, 0 = none. */
+int fho_order_for(double tolerance, int max_order, double half_extent, double distance);
+
+/* bar.rs:30-106 */
+double fho_bar_integral(const double qx[4], const double qy[4], const double qz[4],
+                        double *relative_error);
+void fho_corner_differences(double e1, double e2, double offset, double out[4]);
+/* lines.rs:54-70; returns 0 on the divergent coaxial case. */
+int fho_lines_parallel(double l1, double l2, double offset, double rho, double *out);
+/* lines.rs:135-170 */
+int fho_lines_skew(const double a0[3], const double a1[3], const double b0[3],
+                   const double b1[3], double sin_min, double *out);
+/* aligned.rs:203-232; bars = {length0, length1, width0, width1, height0,
+ * height1, offset0, offset1, offset2}. */
+double fho_aligned_integral(const double bars[9]);
+/* mod.rs:265-276 (closed form when trustworthy, quadrature otherwise). */
+double fho_aligned_dispatch(const double bars[9], int *method);
+
+/* geometry.rs:223-276 + filament.rs:320-546 */
+int fho_discretize_geometry(const double *nodes, size_t n_nodes,
+                            const fho_segment *segments, size_t n_segments,
+                            const fho_grid *grids, fho_filament *out, size_t capacity,
+                            size_t *n_out);
+/* mod.rs:303-369 */
+int fho_mutual(const fho_filament *a, const fho_filament *b, double *value, int *method,
+               int *resolved);
+/* mod.rs:196-202 */
+double fho_self_inductance(const fho_filament *f);
+/* batch.rs:246-253, 355-372: row-major n x n symmetric L. */
+int fho_fill(const fho_filament *f, size_t n, double *L, int threads);
+/* Only the listed pairs (rows[k], cols[k]) -> out[k]. */
+int fho_fill_pairs(const fho_filament *f, size_t n, const int64_t *rows, const int64_t *cols,
+                   size_t count, double *out, int *methods, int threads);
+
+/* mesh.rs:170-255: CSR loop basis.  row_ptr has loops+1 entries; branch and
+ * sign have row_ptr[loops] entries.  Call with NULL outputs to size. */
+int fho_mesh_build(size_t n_nodes, const fho_segment *segments, size_t n_segments,
+                   const size_t *filaments_per_segment, const fho_port *ports, size_t n_ports,
+                   int64_t *row_ptr, int64_t *branch, double *sign, size_t *loops,
+                   size_t *internal, size_t *nnz);
+
+/* solve.rs:1172-1200 congruence M B M^T, B = L (n x n row-major, symmetric)
+ * or B = diag(r) when L == NULL.  out: loops x loops row-major. */
+int fho_congruence(size_t loops, const int64_t *row_ptr, const int64_t *branch,
+                   const double *sign, size_t n, const double *L, const double *r, double *out,
+                   int threads);
+
+/* dense.rs:43-189: x = a^-1 b, column-major, a consumed.  Complex data are
+ * interleaved {re, im}.  Returns FHO_E_SINGULAR on an exact zero pivot. */
+int fho_lu_solve_complex(double *a, size_t order, double *b, size_t nrhs, int threads);
+int fho_lu_solve_real(double *a, size_t order, double *b, size_t nrhs, int threads);
+
+/* Assembled mesh system (solve.rs:680-766). */
+typedef struct fho_system fho_system;
+int fho_system_create(const double *nodes, size_t n_nodes, const fho_segment *segments,
+                      size_t n_segments, const fho_grid *grids, const fho_port *ports,
+                      size_t n_ports, int threads, fho_system **out);
+void fho_system_destroy(fho_system *s);
+/* counts: nodes, segments, filaments, meshes, internal, ports */
+void fho_system_counts(const fho_system *s, size_t counts[6]);
+/* Blocks (row-major): r_ee/l_ee m x m, r_ep/l_ep m x p, r_pp/l_pp p x p. */
+int fho_system_blocks(const fho_system *s, double *r_ee, double *l_ee, double *r_ep,
+                      double *l_ep, double *r_pp, double *l_pp);
+const double *fho_system_inductance(const fho_system *s);
+const fho_filament *fho_system_filaments(const fho_system *s);
+/* solve.rs:806-830 + 1039-1079: z_out = K x p x p complex (row-major,
+ * interleaved {re, im}). */
+int fho_system_sweep(const fho_system *s, const double *freqs, size_t K, double *z_out,
+                     int threads);
+/* One-shot solve.rs:931-938 */
+int fho_solve(const double *nodes, size_t n_nodes, const fho_segment *segments,
+              size_t n_segments, const fho_grid *grids, const fho_port *ports, size_t n_ports,
+              const double *freqs, size_t K, double *z_out, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
