@@ -1,2 +1,15 @@
-"""fasterhenry_amd: MI355X-native PEEC extraction hot path (placeholder init)."""
+"""fasterhenry_amd -- MI355X-native (gfx950) PEEC extraction hot path.
+
+A drop-in for the dense solve path of fasterhenry 0.2.0: the same problem
+types (Geometry, Port, Discretization, SolverChoice,
+DENSE_PATH_MAX_FILAMENTS) and entry points (solve, MeshSystem), computed by
+hand-written HIP kernels in libfasterhenry_gpu.so through its C ABI
+(include/fasterhenry_gpu.h).  Importing the package needs no GPU; the first
+solve loads the native library and raises if it or the device is missing.
+"""
+from .errors import *  # noqa: F401,F403
 from .model import *  # noqa: F401,F403
+from .model import DENSE_PATH_MAX_FILAMENTS, MU0  # noqa: F401
+from .api import (Counts, MeshSystem, SweepResult, Timing, device_math,  # noqa: F401
+                  discretize_geometry, discretize_packed, partial_inductance_matrix, solve,
+                  version)

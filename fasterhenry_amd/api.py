@@ -1,0 +1,252 @@
+"""The library surface: solve(), MeshSystem, SweepResult, kernel-level calls.
+
+Mirrors fasterhenry 0.2.0:
+* ``solve`` -- fasterhenry/src/solve.rs:931-938
+* ``MeshSystem.assemble/impedance/sweep/counts`` -- solve.rs:643-895
+* ``SweepResult`` (schema 1) -- result.rs:47-94
+* ``partial_inductance_matrix`` -- inductance/batch.rs:246-253
+* ``discretize_geometry`` -- solve.rs:947-975
+
+Every call runs on the GPU through libfasterhenry_gpu.so; there is no CPU
+fallback.  ``precision`` selects the factorization arithmetic: "fp64"
+(plain) or "fp32_refined" (FP32 matrix-core factorization + FP64
+refinement).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import native
+from .errors import InvalidArgument, SolveError
+from .model import (DENSE_PATH_MAX_FILAMENTS, FILAMENT_DTYPE, Discretization, Geometry,
+                    PackedProblem, Port, Solver, SolverChoice, pack)
+
+VERSION = "0.1.0"
+
+_PRECISIONS = {"fp64": native.FH_FP64_PLAIN, "fp32_refined": native.FH_FP32_REFINED}
+
+
+def _precision_code(precision: str) -> int:
+    try:
+        return _PRECISIONS[precision]
+    except KeyError:
+        raise InvalidArgument(f"precision must be one of {sorted(_PRECISIONS)}, got {precision!r}")
+
+
+@dataclass
+class Counts:
+    """result.rs:110-125"""
+
+    nodes: int
+    segments: int
+    filaments: int
+    meshes: int
+    internal_meshes: int
+    ports: int
+
+
+@dataclass
+class Timing:
+    """result.rs:129-140 (threads is replaced by the device name)."""
+
+    device: str
+    inductance_s: float
+    assembly_s: float
+    solve_s: float
+    total_s: float
+
+
+@dataclass
+class SweepResult:
+    """Port impedance matrices over frequencies, with provenance (result.rs:47-94)."""
+
+    ports: List[Port]
+    frequencies_hz: List[float]
+    impedance_ohm: np.ndarray  # (K, p, p) complex128
+    counts: Counts
+    timing: Optional[Timing] = None
+    schema_version: int = 1
+    extra: Dict[str, float] = field(default_factory=dict)
+
+    SCHEMA_VERSION = 1
+
+    def resistance(self, index: int) -> np.ndarray:
+        return self.impedance_ohm[index].real.copy()
+
+    def inductance(self, index: int) -> Optional[np.ndarray]:
+        omega = 2.0 * math.pi * self.frequencies_hz[index]
+        return None if omega == 0.0 else self.impedance_ohm[index].imag / omega
+
+    def to_json(self) -> str:
+        """The reference's JSON layout (result.rs:1-37)."""
+        doc = {
+            "schema_version": self.schema_version,
+            "ports": [{"positive": int(p.positive), "negative": int(p.negative),
+                       **({"name": p.name} if p.name is not None else {})} for p in self.ports],
+            "frequencies_hz": [float(f) for f in self.frequencies_hz],
+            "impedance_ohm": [[[{"re": float(z.real), "im": float(z.imag)} for z in row]
+                               for row in m] for m in self.impedance_ohm],
+            "provenance": {
+                "fasterhenry_version": f"fasterhenry_amd {VERSION}",
+                "counts": self.counts.__dict__,
+                **({"timing": self.timing.__dict__} if self.timing else {}),
+            },
+        }
+        return json.dumps(doc)
+
+
+def _counts(handle) -> Counts:
+    c = (C.c_size_t * 6)()
+    native.check(native.lib().fh_system_counts(handle, c))
+    return Counts(*[int(x) for x in c])
+
+
+class MeshSystem:
+    """An assembled mesh system resident in GPU memory (solve.rs:643-895)."""
+
+    def __init__(self, handle, packed: PackedProblem, ports: Sequence[Port], precision: str):
+        self._h = handle
+        self._packed = packed
+        self.ports = list(ports)
+        self.precision = precision
+        self._counts = _counts(handle)
+
+    @classmethod
+    def assemble(cls, geometry: Geometry, ports: Sequence[Port], discretization: Discretization,
+                 precision: str = "fp64", solver: SolverChoice = SolverChoice.DENSE) -> "MeshSystem":
+        packed = pack(geometry, ports, discretization)
+        return cls.assemble_packed(packed, ports, precision, solver)
+
+    @classmethod
+    def assemble_packed(cls, packed: PackedProblem, ports: Sequence[Port], precision: str = "fp64",
+                        solver: SolverChoice = SolverChoice.DENSE) -> "MeshSystem":
+        L = native.lib()
+        h = C.c_void_p()
+        prob = native.problem_struct(packed)
+        native.check(L.fh_system_create(C.byref(prob), solver.value, _precision_code(precision),
+                                        C.byref(h)))
+        return cls(h, packed, ports, precision)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            native.lib().fh_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision: str) -> None:
+        native.check(native.lib().fh_system_set_precision(self._h, _precision_code(precision)))
+        self.precision = precision
+
+    def set_profiling(self, enable: bool) -> None:
+        native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
+
+    def counts(self) -> Counts:
+        return self._counts
+
+    def sweep_raw(self, frequencies) -> tuple:
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        p = self._counts.ports
+        z = np.zeros((len(f), p, p), dtype=np.complex128)
+        info = native.SweepInfo()
+        native.check(native.lib().fh_sweep(self._h, native.ptr(f), len(f), native.ptr(z),
+                                           C.byref(info)))
+        return z, info
+
+    def sweep(self, frequencies) -> SweepResult:
+        """Z at every frequency (solve.rs:881-894)."""
+        z, info = self.sweep_raw(frequencies)
+        timing = Timing("gfx950", info.inductance_s, info.assembly_s, info.solve_s,
+                        info.assembly_s + info.solve_s)
+        return SweepResult(self.ports, [float(x) for x in np.asarray(frequencies, dtype=float)], z,
+                           self._counts, timing,
+                           extra={"refine_steps_max": info.refine_steps_max,
+                                  "refine_change_max": info.refine_change_max})
+
+    def impedance(self, frequency: float) -> np.ndarray:
+        """Z at one frequency (solve.rs:806-830)."""
+        return self.sweep_raw([frequency])[0][0]
+
+    def filaments(self) -> np.ndarray:
+        out = np.zeros(self._counts.filaments, dtype=FILAMENT_DTYPE)
+        native.check(native.lib().fh_system_filaments(self._h, native.ptr(out)))
+        return out
+
+    def inductance_matrix(self) -> np.ndarray:
+        n = self._counts.filaments
+        out = np.zeros((n, n))
+        native.check(native.lib().fh_system_inductance(self._h, native.ptr(out)))
+        return out
+
+    def blocks(self) -> Dict[str, np.ndarray]:
+        m, p = self._counts.internal_meshes, self._counts.ports
+        out = [np.zeros((m, m)), np.zeros((m, m)), np.zeros((m, p)), np.zeros((m, p)),
+               np.zeros((p, p)), np.zeros((p, p))]
+        native.check(native.lib().fh_system_blocks(self._h, *(native.ptr(x) for x in out)))
+        return dict(zip(["r_ee", "l_ee", "r_ep", "l_ep", "r_pp", "l_pp"], out))
+
+
+def solve(geometry: Geometry, ports: Sequence[Port], discretization: Discretization,
+          frequencies, solver: SolverChoice = SolverChoice.AUTO,
+          precision: str = "fp64") -> SweepResult:
+    """Assemble and sweep (solve.rs:931-938).  SolverChoice.AUTO resolves
+    against DENSE_PATH_MAX_FILAMENTS; this build provides the dense path."""
+    n = discretization.filament_count(geometry)
+    if solver.resolve(n) is not Solver.DENSE:
+        raise SolveError(f"{n} filaments resolve to the iterative path, which this build does "
+                         "not provide; pass solver=SolverChoice.DENSE", status=10)
+    system = MeshSystem.assemble(geometry, ports, discretization, precision, SolverChoice.DENSE)
+    try:
+        return system.sweep(frequencies)
+    finally:
+        system.close()
+
+
+def discretize_geometry(geometry: Geometry, discretization: Discretization) -> np.ndarray:
+    """All filaments, segment by segment (solve.rs:947-975), from the native host code."""
+    packed = pack(geometry, [], discretization)
+    return discretize_packed(packed)
+
+
+def discretize_packed(packed: PackedProblem) -> np.ndarray:
+    L = native.lib()
+    prob = native.problem_struct(packed)
+    n = C.c_size_t()
+    native.check(L.fh_discretize(C.byref(prob), None, 0, C.byref(n)))
+    out = np.zeros(n.value, dtype=FILAMENT_DTYPE)
+    native.check(L.fh_discretize(C.byref(prob), native.ptr(out), n.value, C.byref(n)))
+    return out
+
+
+def partial_inductance_matrix(filaments: np.ndarray, with_methods: bool = False):
+    """The symmetric partial-inductance matrix on the GPU (batch.rs:246-253)."""
+    f = np.ascontiguousarray(filaments, dtype=FILAMENT_DTYPE)
+    n = len(f)
+    L = np.zeros((n, n))
+    methods = np.zeros((n, n), dtype=np.int8) if with_methods else None
+    native.check(native.lib().fh_partial_inductance(
+        native.ptr(f), n, native.ptr(L), native.ptr(methods) if with_methods else None))
+    return (L, methods) if with_methods else L
+
+
+def device_math(x) -> tuple:
+    """log and atan as the fill kernels evaluate them on the device."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lg = np.zeros_like(x)
+    at = np.zeros_like(x)
+    native.check(native.lib().fh_device_math(native.ptr(x), len(x), native.ptr(lg), native.ptr(at)))
+    return lg, at
+
+
+def version() -> str:
+    return native.lib().fh_version().decode()

@@ -1,0 +1,100 @@
+// Internal declarations shared by the host and device translation units of
+// libfasterhenry_gpu.so.  Not part of the ABI (include/fasterhenry_gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fasterhenry_gpu.h"
+
+namespace fh {
+
+// Error state of the calling thread, surfaced by fh_last_error /
+// fh_error_detail.
+struct ErrorState {
+  std::string message;
+  int64_t index0 = 0, index1 = 0;
+  double value = 0.0;
+  int32_t code = 0;
+};
+ErrorState &error_state();
+fh_status set_error(fh_status status, const std::string &message, int64_t index0 = 0,
+                    int64_t index1 = 0, double value = 0.0, int32_t code = 0);
+void clear_error();
+
+#define FH_HIP_TRY(expr)                                                                \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::fh::set_error(FH_E_HIP, std::string(#expr " failed: ") +                  \
+                                           hipGetErrorString(e_));                      \
+  } while (0)
+
+// Device buffer owned by a system; freed in its destructor.
+template <typename T>
+struct DeviceArray {
+  T *ptr = nullptr;
+  size_t count = 0;
+  DeviceArray() = default;
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  fh_status allocate(size_t n) {
+    release();
+    if (n == 0) return FH_OK;
+    if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) {
+      ptr = nullptr;
+      (void)hipGetLastError();
+      return set_error(FH_E_NO_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) +
+                                           " bytes failed");
+    }
+    count = n;
+    return FH_OK;
+  }
+};
+
+// Sparse loop basis (mesh.rs:170-255), CSR by loop.
+struct LoopBasis {
+  std::vector<int64_t> row_ptr;  // loops + 1
+  std::vector<int32_t> branch;   // nnz
+  std::vector<double> sign;      // nnz, +-1
+  size_t loops = 0, internal = 0;
+};
+
+// Host geometry (fh_geometry.cpp).
+fh_status validate_geometry(const fh_problem &p);
+fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out);
+fh_status build_loop_basis(const fh_problem &p, const std::vector<size_t> &per_segment,
+                           LoopBasis &out);
+
+// Pair kernel tables in device memory (fh_fill.hip).
+fh_status fill_tables_ready();
+
+// GPU fill (fh_fill.hip): L (n x n row-major, device) of device filaments.
+// methods (device, may be null) receives each entry's method.  On a kernel
+// error returns FH_E_KERNEL with the lowest failing pair.
+fh_status fill_inductance(const fh_filament *d_filaments, size_t n, double *d_L,
+                          int8_t *d_methods, hipStream_t stream);
+
+// GPU congruence (fh_congruence.hip): l_mesh = M L M^T (loops x loops,
+// row-major, device) following solve.rs:1172-1200's summation order.
+fh_status congruence_inductance(const LoopBasis &basis, size_t n, const double *d_L,
+                                double *d_lmesh, hipStream_t stream);
+// Host congruence of the diagonal resistance (solve.rs:728-731): sparse
+// entries (i, j, value), both triangles, i-major.
+void congruence_resistance(const LoopBasis &basis, const std::vector<double> &r,
+                           std::vector<int32_t> &rows, std::vector<int32_t> &cols,
+                           std::vector<double> &vals);
+
+// Device-side portable math check (fh_fill.hip).
+fh_status device_math(const double *x, size_t n, double *log_out, double *atan_out);
+
+}  // namespace fh

@@ -1,0 +1,412 @@
+// The C ABI (include/fasterhenry_gpu.h): assembled mesh systems resident in
+// device memory, frequency sweeps, and the kernel-level entries.
+//
+// fh_system_create follows MeshSystem::assemble_with_coupling
+// (fasterhenry/src/solve.rs:680-766) for the all-pairs coupling; fh_sweep
+// follows MeshSystem::sweep / sweep_impedances (solve.rs:881-894,
+// 1039-1079): frequencies are validated first, then solved in chunks that
+// fit a device-memory budget, and the first failure in list order is
+// reported.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "fh_internal.hpp"
+#include "fh_ldlt.hpp"
+
+using Clock = std::chrono::steady_clock;
+
+namespace fh {
+
+ErrorState &error_state() {
+  thread_local ErrorState state;
+  return state;
+}
+
+fh_status set_error(fh_status status, const std::string &message, int64_t index0, int64_t index1,
+                    double value, int32_t code) {
+  ErrorState &e = error_state();
+  e.message = message;
+  e.index0 = index0;
+  e.index1 = index1;
+  e.value = value;
+  e.code = code;
+  return status;
+}
+
+void clear_error() { error_state() = ErrorState{}; }
+
+}  // namespace fh
+
+using namespace fh;
+
+struct fh_system {
+  hipStream_t stream = nullptr;
+  fh_precision precision = FH_FP64_PLAIN;
+  bool profiling = false;
+  size_t counts[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<fh_filament> filaments;
+  std::vector<double> resistances;
+  LoopBasis basis;
+  DeviceArray<fh_filament> d_filaments;
+  DeviceArray<double> d_L;
+  DeviceArray<double> d_lmesh;
+  std::vector<int32_t> r_rows, r_cols;
+  std::vector<double> r_vals;
+  DeviceArray<int32_t> d_r_rows, d_r_cols;
+  DeviceArray<double> d_r_vals;
+  LdltWorkspace ws;
+  double inductance_s = 0.0, assembly_s = 0.0;
+  ~fh_system() {
+    d_filaments.release();
+    d_L.release();
+    d_lmesh.release();
+    ws.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+fh_status check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return set_error(FH_E_HIP, "no HIP device is visible");
+  }
+  return FH_OK;
+}
+
+size_t sweep_budget_bytes() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return size_t(8) << 30;
+  }
+  // Leave headroom; never ask for more than 96 GiB of workspace.
+  size_t b = free_b / 2;
+  return std::min(b, size_t(96) << 30);
+}
+
+}  // namespace
+
+extern "C" {
+
+fh_solver fh_resolve(fh_solver_choice choice, size_t n) {
+  if (choice == FH_DENSE) return FH_SOLVER_DENSE;
+  if (choice == FH_ITERATIVE) return FH_SOLVER_ITERATIVE;
+  return n <= FH_DENSE_PATH_MAX_FILAMENTS ? FH_SOLVER_DENSE : FH_SOLVER_ITERATIVE;
+}
+
+const char *fh_last_error(void) { return error_state().message.c_str(); }
+
+void fh_error_detail(int64_t *index0, int64_t *index1, double *value, int32_t *code) {
+  const ErrorState &e = error_state();
+  if (index0) *index0 = e.index0;
+  if (index1) *index1 = e.index1;
+  if (value) *value = e.value;
+  if (code) *code = e.code;
+}
+
+const char *fh_version(void) { return "fasterhenry_amd 0.1.0 (gfx950)"; }
+
+fh_status fh_discretize(const fh_problem *problem, fh_filament *out, size_t capacity,
+                        size_t *n_out) {
+  clear_error();
+  if (!problem || !n_out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  std::vector<fh_filament> f;
+  fh_status st = discretize_problem(*problem, f);
+  if (st) return st;
+  *n_out = f.size();
+  if (out) {
+    if (capacity < f.size()) return set_error(FH_E_INVALID_ARGUMENT, "capacity too small");
+    std::memcpy(out, f.data(), f.size() * sizeof(fh_filament));
+  }
+  return FH_OK;
+}
+
+fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
+                           fh_precision precision, fh_system **out) {
+  clear_error();
+  if (!problem || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (precision != FH_FP64_PLAIN && precision != FH_FP32_REFINED)
+    return set_error(FH_E_INVALID_ARGUMENT, "unknown precision");
+  const auto t0 = Clock::now();
+  size_t nfil = 0;
+  for (size_t s = 0; s < problem->n_segments && problem->grids; ++s)
+    nfil += static_cast<size_t>(problem->grids[s].nw) * problem->grids[s].nh;
+  if (fh_resolve(choice, nfil) != FH_SOLVER_DENSE)
+    return set_error(FH_E_UNSUPPORTED,
+                     "the iterative (pFFT + GMRES) path is not provided by this build; "
+                     "choose FH_DENSE");
+  fh_status st = check_device();
+  if (st) return st;
+  std::unique_ptr<fh_system> s(new fh_system());
+  s->precision = precision;
+  if ((st = discretize_problem(*problem, s->filaments))) return st;
+  const size_t n = s->filaments.size();
+  std::vector<size_t> per(problem->n_segments);
+  for (size_t q = 0; q < problem->n_segments; ++q)
+    per[q] = static_cast<size_t>(problem->grids[q].nw) * problem->grids[q].nh;
+  if ((st = build_loop_basis(*problem, per, s->basis))) return st;
+  s->resistances.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const fh_filament &f = s->filaments[i];
+    s->resistances[i] = f.length / (f.sigma * (f.width * f.height));  // solve.rs:941-943
+  }
+  FH_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if ((st = s->d_filaments.allocate(n))) return st;
+  if ((st = s->d_L.allocate(n * n))) return st;
+  if (n) {
+    FH_HIP_TRY(hipMemcpyAsync(s->d_filaments.ptr, s->filaments.data(), n * sizeof(fh_filament),
+                              hipMemcpyHostToDevice, s->stream));
+  }
+  const auto ti = Clock::now();
+  if ((st = fill_inductance(s->d_filaments.ptr, n, s->d_L.ptr, nullptr, s->stream))) return st;
+  s->inductance_s = seconds_since(ti);
+  const size_t loops = s->basis.loops;
+  if ((st = s->d_lmesh.allocate(loops * loops))) return st;
+  if ((st = congruence_inductance(s->basis, n, s->d_L.ptr, s->d_lmesh.ptr, s->stream))) return st;
+  congruence_resistance(s->basis, s->resistances, s->r_rows, s->r_cols, s->r_vals);
+  const size_t nnz = s->r_vals.size();
+  if ((st = s->d_r_rows.allocate(nnz)) || (st = s->d_r_cols.allocate(nnz)) ||
+      (st = s->d_r_vals.allocate(nnz)))
+    return st;
+  if (nnz) {
+    FH_HIP_TRY(hipMemcpyAsync(s->d_r_rows.ptr, s->r_rows.data(), nnz * sizeof(int32_t),
+                              hipMemcpyHostToDevice, s->stream));
+    FH_HIP_TRY(hipMemcpyAsync(s->d_r_cols.ptr, s->r_cols.data(), nnz * sizeof(int32_t),
+                              hipMemcpyHostToDevice, s->stream));
+    FH_HIP_TRY(hipMemcpyAsync(s->d_r_vals.ptr, s->r_vals.data(), nnz * sizeof(double),
+                              hipMemcpyHostToDevice, s->stream));
+  }
+  FH_HIP_TRY(hipStreamSynchronize(s->stream));
+  s->counts[0] = problem->n_nodes;
+  s->counts[1] = problem->n_segments;
+  s->counts[2] = n;
+  s->counts[3] = loops;
+  s->counts[4] = s->basis.internal;
+  s->counts[5] = loops - s->basis.internal;
+  s->assembly_s = seconds_since(t0);
+  *out = s.release();
+  return FH_OK;
+}
+
+void fh_system_destroy(fh_system *system) { delete system; }
+
+fh_status fh_system_set_precision(fh_system *s, fh_precision precision) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  if (precision != FH_FP64_PLAIN && precision != FH_FP32_REFINED)
+    return set_error(FH_E_INVALID_ARGUMENT, "unknown precision");
+  s->precision = precision;
+  return FH_OK;
+}
+
+fh_status fh_system_set_profiling(fh_system *s, int enable) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  s->profiling = enable != 0;
+  return FH_OK;
+}
+
+fh_status fh_system_counts(const fh_system *s, size_t counts[6]) {
+  clear_error();
+  if (!s || !counts) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  std::memcpy(counts, s->counts, sizeof(s->counts));
+  return FH_OK;
+}
+
+fh_status fh_system_filaments(const fh_system *s, fh_filament *out) {
+  clear_error();
+  if (!s || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  std::memcpy(out, s->filaments.data(), s->filaments.size() * sizeof(fh_filament));
+  return FH_OK;
+}
+
+fh_status fh_system_inductance(const fh_system *s, double *L_out) {
+  clear_error();
+  if (!s || !L_out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  const size_t n = s->filaments.size();
+  if (n) FH_HIP_TRY(hipMemcpy(L_out, s->d_L.ptr, n * n * sizeof(double), hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
+fh_status fh_system_blocks(const fh_system *s, double *r_ee, double *l_ee, double *r_ep,
+                           double *l_ep, double *r_pp, double *l_pp) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  const size_t loops = s->basis.loops, m = s->basis.internal, p = loops - m;
+  std::vector<double> lm(loops * loops), rm(loops * loops, 0.0);
+  if (loops)
+    FH_HIP_TRY(hipMemcpy(lm.data(), s->d_lmesh.ptr, lm.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < s->r_vals.size(); ++e)
+    rm[static_cast<size_t>(s->r_rows[e]) * loops + s->r_cols[e]] = s->r_vals[e];
+  for (size_t i = 0; i < loops; ++i)
+    for (size_t j = 0; j < loops; ++j) {
+      const double r = rm[i * loops + j], l = lm[i * loops + j];
+      if (i < m && j < m) {
+        if (r_ee) r_ee[i * m + j] = r;
+        if (l_ee) l_ee[i * m + j] = l;
+      } else if (i < m) {
+        if (r_ep) r_ep[i * p + (j - m)] = r;
+        if (l_ep) l_ep[i * p + (j - m)] = l;
+      } else if (j >= m) {
+        if (r_pp) r_pp[(i - m) * p + (j - m)] = r;
+        if (l_pp) l_pp[(i - m) * p + (j - m)] = l;
+      }
+    }
+  return FH_OK;
+}
+
+fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
+                   fh_sweep_info *info) {
+  clear_error();
+  if (!s || (K && (!freqs || !z_out))) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  for (size_t k = 0; k < K; ++k)
+    if (!(std::isfinite(freqs[k]) && freqs[k] >= 0.0))
+      return set_error(FH_E_INVALID_FREQUENCY,
+                       "frequency " + std::to_string(k) + " is " + std::to_string(freqs[k]) +
+                           " Hz; frequencies must be finite and non-negative",
+                       static_cast<int64_t>(k), 0, freqs[k]);
+  const auto t0 = Clock::now();
+  const size_t m = s->basis.internal, p = s->basis.loops - m;
+  Profiler prof;
+  Profiler *pp = s->profiling ? &prof : nullptr;
+  if (m == 0) {
+    // No internal loops: Z = Zpp, nothing to solve (solve.rs:1159-1161).
+    std::vector<double> lm(p * p);
+    if (p)
+      FH_HIP_TRY(hipMemcpy(lm.data(), s->d_lmesh.ptr, lm.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> rm(p * p, 0.0);
+    for (size_t e = 0; e < s->r_vals.size(); ++e) rm[s->r_rows[e] * p + s->r_cols[e]] = s->r_vals[e];
+    for (size_t k = 0; k < K; ++k) {
+      const double omega = 6.283185307179586 * freqs[k];
+      for (size_t q = 0; q < p * p; ++q) {
+        z_out[2 * (k * p * p + q)] = rm[q];
+        z_out[2 * (k * p * p + q) + 1] = freqs[k] == 0.0 ? 0.0 : omega * lm[q];
+      }
+    }
+  } else {
+    const size_t per = ldlt_bytes_per_frequency(m, p);
+    size_t chunk = std::max<size_t>(1, sweep_budget_bytes() / per);
+    chunk = std::min(chunk, K);
+    DeviceArray<double> d_omegas, d_z;
+    DeviceArray<int> d_sing;
+    fh_status st;
+    if ((st = d_omegas.allocate(std::max<size_t>(chunk, 1))) ||
+        (st = d_z.allocate(std::max<size_t>(chunk * p * p * 2, 1))) ||
+        (st = d_sing.allocate(std::max<size_t>(chunk, 1))))
+      return st;
+    DenseSolveInputs in;
+    in.m = m;
+    in.p = p;
+    in.d_lmesh = s->d_lmesh.ptr;
+    in.d_r_rows = s->d_r_rows.ptr;
+    in.d_r_cols = s->d_r_cols.ptr;
+    in.d_r_vals = s->d_r_vals.ptr;
+    in.r_nnz = s->r_vals.size();
+    std::vector<double> omegas(chunk);
+    std::vector<int> sing(chunk);
+    for (size_t c0 = 0; c0 < K; c0 += chunk) {
+      const size_t kc = std::min(chunk, K - c0);
+      for (size_t q = 0; q < kc; ++q) omegas[q] = 6.283185307179586 * freqs[c0 + q];
+      FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double),
+                                hipMemcpyHostToDevice, s->stream));
+      FH_HIP_TRY(hipMemsetAsync(d_sing.ptr, 0, kc * sizeof(int), s->stream));
+      if ((st = ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, d_sing.ptr, s->ws,
+                                 s->stream, pp)))
+        return st;
+      FH_HIP_TRY(hipMemcpyAsync(z_out + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, s->stream));
+      FH_HIP_TRY(hipMemcpyAsync(sing.data(), d_sing.ptr, kc * sizeof(int), hipMemcpyDeviceToHost,
+                                s->stream));
+      FH_HIP_TRY(hipStreamSynchronize(s->stream));
+      for (size_t q = 0; q < kc; ++q) {
+        bool bad = sing[q] != 0;
+        for (size_t e = 0; e < 2 * p * p && !bad; ++e)
+          if (!std::isfinite(z_out[2 * (c0 + q) * p * p + e])) bad = true;
+        if (bad)
+          return set_error(FH_E_SINGULAR,
+                           "the mesh system is singular or ill-conditioned at " +
+                               std::to_string(freqs[c0 + q]) + " Hz",
+                           static_cast<int64_t>(c0 + q), 0, freqs[c0 + q]);
+      }
+    }
+  }
+  if (info) {
+    info->nodes = s->counts[0];
+    info->segments = s->counts[1];
+    info->filaments = s->counts[2];
+    info->meshes = s->counts[3];
+    info->internal_meshes = s->counts[4];
+    info->ports = s->counts[5];
+    info->inductance_s = s->inductance_s;
+    info->assembly_s = s->assembly_s;
+    info->solve_s = seconds_since(t0);
+    info->refine_steps_max = 0;
+    info->refine_change_max = 0.0;
+    info->update_kernel_ms = pp ? prof.total_ms() : 0.0;
+    info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used) : 0;
+    info->update_kernel_flops = pp ? prof.flops : 0.0;
+  }
+  return FH_OK;
+}
+
+fh_status fh_impedance(fh_system *s, double f, double *z_out) {
+  return fh_sweep(s, &f, 1, z_out, nullptr);
+}
+
+fh_status fh_solve(const fh_problem *problem, fh_solver_choice choice, fh_precision precision,
+                   const double *freqs, size_t K, double *z_out, fh_sweep_info *info) {
+  fh_system *s = nullptr;
+  fh_status st = fh_system_create(problem, choice, precision, &s);
+  if (st) return st;
+  st = fh_sweep(s, freqs, K, z_out, info);
+  fh_system_destroy(s);
+  return st;
+}
+
+fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *L_out,
+                                int8_t *methods) {
+  clear_error();
+  if ((n && (!filaments || !L_out))) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (n == 0) return FH_OK;
+  fh_status st = check_device();
+  if (st) return st;
+  hipStream_t stream;
+  FH_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{stream};
+  DeviceArray<fh_filament> d_f;
+  DeviceArray<double> d_L;
+  DeviceArray<int8_t> d_m;
+  if ((st = d_f.allocate(n)) || (st = d_L.allocate(n * n))) return st;
+  if (methods && (st = d_m.allocate(n * n))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(d_f.ptr, filaments, n * sizeof(fh_filament), hipMemcpyHostToDevice, stream));
+  if ((st = fill_inductance(d_f.ptr, n, d_L.ptr, methods ? d_m.ptr : nullptr, stream))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(L_out, d_L.ptr, n * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  if (methods)
+    FH_HIP_TRY(hipMemcpyAsync(methods, d_m.ptr, n * n, hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));
+  return FH_OK;
+}
+
+fh_status fh_device_math(const double *x, size_t n, double *log_out, double *atan_out) {
+  clear_error();
+  if (n && (!x || !log_out || !atan_out)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  fh_status st = check_device();
+  if (st) return st;
+  return device_math(x, n, log_out, atan_out);
+}
+
+}  // extern "C"

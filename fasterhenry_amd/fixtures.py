@@ -12,6 +12,7 @@
 """
 from __future__ import annotations
 
+import math
 from typing import List, Tuple
 
 import numpy as np
@@ -163,3 +164,56 @@ def log_frequencies(f0: float, f1: float, count: int) -> np.ndarray:
         return np.array([f0])
     k = np.arange(count, dtype=np.float64)
     return f0 * (f1 / f0) ** (k / (count - 1))
+
+
+def lcg(seed: int):
+    """Uniform deviates in [0, 1) (fasterhenry/src/pfft/cloud_draws.rs:36-44)."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def draw() -> float:
+        nonlocal state
+        state = (state * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return (state >> 11) / float(1 << 53)
+
+    return draw
+
+
+def cloud_side(n: int) -> float:
+    """cloud_draws.rs:54-56"""
+    return 10e-3 * float(np.cbrt(n / 400.0))
+
+
+def random_bars(n: int, side: float, manhattan: bool, seed: int):
+    """cloud_draws.rs:69-95: bars of length 0.5-1.5 mm in a cube; Rust
+    evaluates an assignment's value before its place, so the sign draw
+    precedes the axis draw."""
+    r = lcg(seed)
+    out = []
+    for _ in range(n):
+        a = [side * r(), side * r(), side * r()]
+        length = 1e-3 * (0.5 + r())
+        if manhattan:
+            d = [0.0, 0.0, 0.0]
+            sign = -1.0 if r() < 0.5 else 1.0
+            d[min(int(r() * 3.0), 2)] = sign
+        else:
+            v = [r() - 0.5, r() - 0.5, r() - 0.5]
+            nrm = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+            d = [c / nrm for c in v]
+        b = [a[k] + length * d[k] for k in range(3)]
+        width = 0.1e-3 + 0.2e-3 * r()
+        height = 0.05e-3 + 0.1e-3 * r()
+        out.append((a, b, width, height))
+    return out
+
+
+def random_cloud_problem(n: int, manhattan: bool, seed: int, side: float = None):
+    """The bars of random_bars as a geometry of single-filament segments
+    (benches/support.rs:147-163 random_set), with one port on bar 0."""
+    side = cloud_side(n) if side is None else side
+    g = Geometry()
+    for (a, b, w, h) in random_bars(n, side, manhattan, seed):
+        na = g.add_node(Node(*a))
+        nb = g.add_node(Node(*b))
+        g.add_segment(SegmentDef(na, nb, w, h, COPPER))
+    return g, [Port(0, 1)], Discretization.uniform(1, 1)

@@ -1,0 +1,141 @@
+"""ctypes binding of libfasterhenry_gpu.so (include/fasterhenry_gpu.h).
+
+The library is built in-tree (fasterhenry_amd/lib) by
+``__graft_entry__.build()`` or ``make -C fasterhenry_amd/csrc``.  There is no
+fallback: if the library is missing, or no HIP device is visible, every
+entry raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import errors
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfasterhenry_gpu.so")
+
+FH_OK = 0
+FH_FP64_PLAIN = 0
+FH_FP32_REFINED = 1
+FH_DENSE_PATH_MAX_FILAMENTS = 10000
+
+SYMBOLS = [
+    "fh_resolve", "fh_system_create", "fh_sweep", "fh_impedance", "fh_system_set_precision",
+    "fh_system_set_profiling", "fh_system_counts", "fh_system_filaments",
+    "fh_system_inductance", "fh_system_blocks", "fh_system_destroy", "fh_solve",
+    "fh_partial_inductance", "fh_discretize", "fh_device_math", "fh_last_error",
+    "fh_error_detail", "fh_version",
+]
+
+_ERRORS = {
+    1: errors.InvalidArgument, 2: errors.GeometryError, 3: errors.DiscretizeError,
+    4: errors.KernelError, 5: errors.MeshError, 6: errors.InvalidGrading,
+    7: errors.InvalidFrequency, 8: errors.Singular, 9: errors.NotConverged,
+    10: errors.SolveError, 11: errors.SolveError, 12: errors.HipError,
+}
+
+
+class Problem(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("n_nodes", C.c_size_t), ("segments", C.c_void_p),
+                ("grids", C.c_void_p), ("n_segments", C.c_size_t), ("ports", C.c_void_p),
+                ("n_ports", C.c_size_t)]
+
+
+class SweepInfo(C.Structure):
+    _fields_ = [("nodes", C.c_size_t), ("segments", C.c_size_t), ("filaments", C.c_size_t),
+                ("meshes", C.c_size_t), ("internal_meshes", C.c_size_t), ("ports", C.c_size_t),
+                ("inductance_s", C.c_double), ("assembly_s", C.c_double),
+                ("solve_s", C.c_double), ("refine_steps_max", C.c_int32),
+                ("reserved", C.c_int32), ("refine_change_max", C.c_double),
+                ("update_kernel_ms", C.c_double), ("update_kernel_launches", C.c_int64),
+                ("update_kernel_flops", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
+                "`make -C fasterhenry_amd/csrc`; there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, dp = C.c_void_p, C.c_size_t, C.POINTER(C.c_double)
+        st = C.c_int
+        L.fh_resolve.restype = C.c_int
+        L.fh_resolve.argtypes = [C.c_int, sz]
+        L.fh_system_create.restype = st
+        L.fh_system_create.argtypes = [C.POINTER(Problem), C.c_int, C.c_int, C.POINTER(vp)]
+        L.fh_sweep.restype = st
+        L.fh_sweep.argtypes = [vp, vp, sz, vp, C.POINTER(SweepInfo)]
+        L.fh_impedance.restype = st
+        L.fh_impedance.argtypes = [vp, C.c_double, vp]
+        L.fh_system_set_precision.restype = st
+        L.fh_system_set_precision.argtypes = [vp, C.c_int]
+        L.fh_system_set_profiling.restype = st
+        L.fh_system_set_profiling.argtypes = [vp, C.c_int]
+        L.fh_system_counts.restype = st
+        L.fh_system_counts.argtypes = [vp, C.POINTER(sz)]
+        L.fh_system_filaments.restype = st
+        L.fh_system_filaments.argtypes = [vp, vp]
+        L.fh_system_inductance.restype = st
+        L.fh_system_inductance.argtypes = [vp, vp]
+        L.fh_system_blocks.restype = st
+        L.fh_system_blocks.argtypes = [vp] * 7
+        L.fh_system_destroy.restype = None
+        L.fh_system_destroy.argtypes = [vp]
+        L.fh_solve.restype = st
+        L.fh_solve.argtypes = [C.POINTER(Problem), C.c_int, C.c_int, vp, sz, vp,
+                               C.POINTER(SweepInfo)]
+        L.fh_partial_inductance.restype = st
+        L.fh_partial_inductance.argtypes = [vp, sz, vp, vp]
+        L.fh_discretize.restype = st
+        L.fh_discretize.argtypes = [C.POINTER(Problem), vp, sz, C.POINTER(sz)]
+        L.fh_device_math.restype = st
+        L.fh_device_math.argtypes = [vp, sz, vp, vp]
+        L.fh_last_error.restype = C.c_char_p
+        L.fh_last_error.argtypes = []
+        L.fh_error_detail.restype = None
+        L.fh_error_detail.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), dp,
+                                      C.POINTER(C.c_int32)]
+        L.fh_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    """Raise the SolveError subclass matching a non-zero fh_status."""
+    if status == FH_OK:
+        return
+    L = lib()
+    msg = L.fh_last_error().decode(errors="replace")
+    i0, i1, v, code = C.c_int64(), C.c_int64(), C.c_double(), C.c_int32()
+    L.fh_error_detail(C.byref(i0), C.byref(i1), C.byref(v), C.byref(code))
+    cls = _ERRORS.get(status, errors.SolveError)
+    raise cls(msg or f"fh_status {status}", status=status, index0=i0.value, index1=i1.value,
+              value=v.value, code=code.value)
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def problem_struct(packed) -> Problem:
+    p = Problem()
+    p.nodes = ptr(packed.nodes) if len(packed.nodes) else None
+    p.n_nodes = len(packed.nodes)
+    p.segments = ptr(packed.segments) if len(packed.segments) else None
+    p.grids = ptr(packed.grids) if len(packed.grids) else None
+    p.n_segments = len(packed.segments)
+    p.ports = ptr(packed.ports) if len(packed.ports) else None
+    p.n_ports = len(packed.ports)
+    return p

@@ -1,0 +1,193 @@
+/*
+ * fasterhenry_gpu.h -- C ABI of the MI355X (gfx950) extraction hot path.
+ *
+ * A drop-in for the dense solve path of fasterhenry 0.2.0.  The reference is
+ * a safe Rust library without an FFI; each entry below names the reference
+ * item it replaces (paths relative to the reference repository root).  The
+ * ABI is plain C: fixed-width integers, doubles and pointers, caller-owned
+ * inputs and outputs, no C++ or HIP types.  A Rust caller binds it with
+ * `extern "C"` and `#[repr(C)]` mirrors of the structs.
+ *
+ * Threading: one fh_system per thread at a time; distinct systems may be
+ * used concurrently (each owns its HIP stream).  Results are deterministic:
+ * no reduction on the device depends on scheduling order.
+ */
+#ifndef FASTERHENRY_GPU_H
+#define FASTERHENRY_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* DENSE_PATH_MAX_FILAMENTS (fasterhenry/src/solve.rs:133). */
+#define FH_DENSE_PATH_MAX_FILAMENTS 10000u
+
+/* Status codes.  They mirror SolveError (fasterhenry/src/solve.rs:567-637),
+ * KernelError (inductance/mod.rs:93-115), MeshError (mesh.rs:99-148) and
+ * GeometryError (geometry.rs).  fh_last_error() and fh_error_detail() give
+ * the message and the indices. */
+typedef enum fh_status {
+  FH_OK = 0,
+  FH_E_INVALID_ARGUMENT = 1, /* null pointer, bad size or option */
+  FH_E_GEOMETRY = 2,         /* GeometryError: index0 = node or segment */
+  FH_E_DISCRETIZE = 3,       /* SolveError::Discretize: index0 = segment */
+  FH_E_KERNEL = 4,           /* SolveError::Kernel: index0, index1 = filament pair */
+  FH_E_MESH = 5,             /* SolveError::Mesh: code = which MeshError */
+  FH_E_INVALID_GRADING = 6,  /* SolveError::InvalidGrading: index0 = segment */
+  FH_E_INVALID_FREQUENCY = 7,/* SolveError::InvalidFrequency: index0, value */
+  FH_E_SINGULAR = 8,         /* SolveError::Singular: index0 = frequency index, value = Hz */
+  FH_E_NOT_CONVERGED = 9,    /* refinement missed 1e-10: index0 = frequency index */
+  FH_E_UNSUPPORTED = 10,     /* a path this build does not provide */
+  FH_E_NO_MEMORY = 11,       /* host or device allocation failed */
+  FH_E_HIP = 12              /* HIP runtime error */
+} fh_status;
+
+/* SolverChoice (solve.rs:169-189) and Solver (solve.rs:143-158). */
+typedef enum fh_solver_choice { FH_AUTO = 0, FH_DENSE = 1, FH_ITERATIVE = 2 } fh_solver_choice;
+typedef enum fh_solver { FH_SOLVER_DENSE = 0, FH_SOLVER_ITERATIVE = 1 } fh_solver;
+
+/* Arithmetic of the per-frequency factorization.  FH_FP64_PLAIN: the whole
+ * solve in FP64.  FH_FP32_REFINED: the factorization in FP32 on the matrix
+ * cores, FP64 iterative refinement of the solution to a 1e-10 relative
+ * Schur-complement change, FP64 everywhere else. */
+typedef enum fh_precision { FH_FP64_PLAIN = 0, FH_FP32_REFINED = 1 } fh_precision;
+
+/* SolverChoice::resolve (solve.rs:179-189). */
+fh_solver fh_resolve(fh_solver_choice choice, size_t n_filaments);
+
+/* Geometry.  SegmentDef (geometry.rs:289-330): nodes a, b; cross-section
+ * width x height; conductivity sigma; optional width direction. */
+typedef struct fh_segment {
+  uint32_t a, b;
+  double width, height, sigma;
+  double width_dir[3];
+  uint32_t has_width_dir;
+  uint32_t reserved;
+} fh_segment;
+
+/* A segment's resolved grid: Discretization::resolve (solve.rs:468-510)
+ * gives one per segment.  graded = 0 cuts with discretize (filament.rs:320),
+ * graded = 1 with discretize_graded_per_axis (filament.rs:478). */
+typedef struct fh_grid {
+  uint32_t nw, nh;
+  double width_ratio, height_ratio;
+  uint32_t graded;
+  uint32_t reserved;
+} fh_grid;
+
+/* Port (mesh.rs:68-94). */
+typedef struct fh_port {
+  uint32_t positive, negative;
+} fh_port;
+
+/* A problem: nodes (xyz triples), segments with one grid each, ports. */
+typedef struct fh_problem {
+  const double *nodes;
+  size_t n_nodes;
+  const fh_segment *segments;
+  const fh_grid *grids;
+  size_t n_segments;
+  const fh_port *ports;
+  size_t n_ports;
+} fh_problem;
+
+/* A filament (filament.rs:34-140): centreline, frame, cross-section. */
+typedef struct fh_filament {
+  double start[3], end[3];
+  double length;
+  double dir[3], wdir[3], hdir[3];
+  double width, height, sigma;
+} fh_filament;
+
+/* Counts (result.rs:110-125) plus timing (result.rs:129-140) of a sweep. */
+typedef struct fh_sweep_info {
+  size_t nodes, segments, filaments, meshes, internal_meshes, ports;
+  double inductance_s; /* fill of L */
+  double assembly_s;   /* discretization + fill + congruence */
+  double solve_s;      /* the sweep, from frequencies in to Z out */
+  /* FH_FP32_REFINED: worst refinement steps and final relative Schur change */
+  int32_t refine_steps_max;
+  int32_t reserved;
+  double refine_change_max;
+  /* instrumentation (fh_system_set_profiling): time and launch count of
+   * the dominant kernel family (the trailing update), HIP events */
+  double update_kernel_ms;
+  int64_t update_kernel_launches;
+  double update_kernel_flops; /* algorithmic flops those launches did */
+} fh_sweep_info;
+
+typedef struct fh_system fh_system;
+
+/* MeshSystem::assemble (solve.rs:680-766): discretize, build the loop basis,
+ * fill L on the GPU, reduce to mesh blocks on the GPU; the blocks stay
+ * resident in device memory until fh_system_destroy.  Only the dense path
+ * is provided: choice = FH_ITERATIVE, or FH_AUTO above
+ * FH_DENSE_PATH_MAX_FILAMENTS, returns FH_E_UNSUPPORTED. */
+fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
+                           fh_precision precision, fh_system **out);
+
+/* MeshSystem::sweep (solve.rs:881-894, 1039-1079): z_out receives
+ * K x p x p complex values, row-major, each as {re, im}.  f = 0 is solved
+ * in real arithmetic with Im Z exactly 0 (solve.rs:809-812).  The first
+ * failing frequency in list order is reported.  info may be NULL. */
+fh_status fh_sweep(fh_system *system, const double *freqs_hz, size_t K, double *z_out,
+                   fh_sweep_info *info);
+
+/* MeshSystem::impedance (solve.rs:806-830) at one frequency. */
+fh_status fh_impedance(fh_system *system, double freq_hz, double *z_out);
+
+/* Changes the factorization arithmetic of an assembled system. */
+fh_status fh_system_set_precision(fh_system *system, fh_precision precision);
+
+/* Brackets each launch of the dominant kernel with HIP events on the
+ * system's stream; totals are reported in fh_sweep_info. */
+fh_status fh_system_set_profiling(fh_system *system, int enable);
+
+/* Counts (result.rs:110-125): nodes, segments, filaments, meshes,
+ * internal meshes, ports. */
+fh_status fh_system_counts(const fh_system *system, size_t counts[6]);
+
+/* Copies of the resident data, for parity checks.  Any pointer may be NULL.
+ * filaments: n; L: n x n row-major; mesh blocks row-major: l_ee m x m,
+ * l_ep m x p, l_pp p x p, r_ee m x m, r_ep m x p, r_pp p x p. */
+fh_status fh_system_filaments(const fh_system *system, fh_filament *out);
+fh_status fh_system_inductance(const fh_system *system, double *L_out);
+fh_status fh_system_blocks(const fh_system *system, double *r_ee, double *l_ee, double *r_ep,
+                           double *l_ep, double *r_pp, double *l_pp);
+
+void fh_system_destroy(fh_system *system);
+
+/* solve() (solve.rs:931-938): create + sweep + destroy. */
+fh_status fh_solve(const fh_problem *problem, fh_solver_choice choice, fh_precision precision,
+                   const double *freqs_hz, size_t K, double *z_out, fh_sweep_info *info);
+
+/* Kernel level.  partial_inductance_matrix (inductance/batch.rs:246-253):
+ * L_out (n x n, row-major, host) for host filaments.  methods (may be NULL)
+ * receives the Method of inductance/mod.rs:130-152 per entry (0 orthogonal,
+ * 1 bar closed form, 2 aligned quadrature, 3 point quadrature, 4 sampled). */
+fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *L_out,
+                                int8_t *methods);
+
+/* discretize_geometry (solve.rs:947-975) on the host: n_out receives the
+ * filament count; out (capacity entries) may be NULL to query the count. */
+fh_status fh_discretize(const fh_problem *problem, fh_filament *out, size_t capacity,
+                        size_t *n_out);
+
+/* Device-side log and atan of the fill kernels, for bit-level checks. */
+fh_status fh_device_math(const double *x, size_t n, double *log_out, double *atan_out);
+
+/* Message of the last error on the calling thread ("" when none). */
+const char *fh_last_error(void);
+/* Indices and value attached to the last error: index0, index1, value, code. */
+void fh_error_detail(int64_t *index0, int64_t *index1, double *value, int32_t *code);
+
+/* Library version string. */
+const char *fh_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
