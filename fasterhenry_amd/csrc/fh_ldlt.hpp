@@ -13,9 +13,16 @@ namespace fh {
 struct DenseSolveInputs {
   size_t m = 0, p = 0;              // internal loops, ports
   const double *d_lmesh = nullptr;  // (m + p) x (m + p), row-major (symmetric)
-  const int32_t *d_r_rows = nullptr, *d_r_cols = nullptr;  // r_mesh, sparse, both triangles
+  // r_mesh, sparse, both triangles (for forming R + jwL)
+  const int32_t *d_r_rows = nullptr, *d_r_cols = nullptr;
   const double *d_r_vals = nullptr;
   size_t r_nnz = 0;
+  // r_ee as CSR over its m rows (refinement residuals), r_ep dense [q][i]
+  const int64_t *d_ree_ptr = nullptr;
+  const int32_t *d_ree_col = nullptr;
+  const double *d_ree_val = nullptr;
+  const double *d_rep = nullptr;
+  const double *d_rpp = nullptr;  // r_pp dense p x p
 };
 
 // Grow-only device workspace.
@@ -83,13 +90,26 @@ struct Profiler {
   }
 };
 
-// FP64 factorization of K frequencies at once; d_z receives K x p x p
-// complex values ({re, im}); d_singular[f] is set to 1 on an exact zero
-// pivot (it must be cleared by the caller).
+// Outcome of a batch solve.
+struct BatchResult {
+  std::vector<int> singular;         // per frequency: exact zero pivot
+  std::vector<int> refine_steps;     // per frequency (refined form)
+  std::vector<double> refine_change; // per frequency: final |x^T r| / |b^T x|
+  std::vector<int> converged;        // per frequency (refined form)
+};
+
+// Plain FP64: Z of K frequencies at once into d_z (K x p x p, {re, im}).
 fh_status ldlt_solve_batch(const DenseSolveInputs &in, const double *d_omegas, int K, double *d_z,
-                           int *d_singular, LdltWorkspace &ws, hipStream_t stream, Profiler *prof);
+                           BatchResult &out, LdltWorkspace &ws, hipStream_t stream,
+                           Profiler *prof);
+
+// FP32 factorization on the matrix cores + FP64 iterative refinement of
+// x = Zee^-1 Zep, then Z = Zpp - Zep^T x - x^T r (r the final residual).
+fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas, int K,
+                             double *d_z, BatchResult &out, LdltWorkspace &ws,
+                             hipStream_t stream, Profiler *prof);
 
 // Device bytes one frequency of a batch needs.
-size_t ldlt_bytes_per_frequency(size_t m, size_t p);
+size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined);
 
 }  // namespace fh

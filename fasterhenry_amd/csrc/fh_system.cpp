@@ -57,6 +57,10 @@ struct fh_system {
   std::vector<double> r_vals;
   DeviceArray<int32_t> d_r_rows, d_r_cols;
   DeviceArray<double> d_r_vals;
+  // r_ee as CSR (refinement residuals), r_ep dense [q][i], r_pp dense
+  DeviceArray<int64_t> d_ree_ptr;
+  DeviceArray<int32_t> d_ree_col;
+  DeviceArray<double> d_ree_val, d_rep, d_rpp;
   LdltWorkspace ws;
   double inductance_s = 0.0, assembly_s = 0.0;
   ~fh_system() {
@@ -187,7 +191,49 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
     FH_HIP_TRY(hipMemcpyAsync(s->d_r_vals.ptr, s->r_vals.data(), nnz * sizeof(double),
                               hipMemcpyHostToDevice, s->stream));
   }
-  FH_HIP_TRY(hipStreamSynchronize(s->stream));
+  {
+    // CSR of the r_ee block, dense r_ep ([q][i]) and r_pp, for the refined form.
+    const size_t m = s->basis.internal, p = loops - m;
+    std::vector<std::vector<std::pair<int32_t, double>>> rows(m);
+    std::vector<double> rep(m * p, 0.0), rpp(p * p, 0.0);
+    for (size_t e = 0; e < nnz; ++e) {
+      const size_t i = static_cast<size_t>(s->r_rows[e]), j = static_cast<size_t>(s->r_cols[e]);
+      if (i < m && j < m) rows[i].emplace_back(static_cast<int32_t>(j), s->r_vals[e]);
+      else if (i < m && j >= m) rep[(j - m) * m + i] = s->r_vals[e];
+      else if (i >= m && j >= m) rpp[(i - m) * p + (j - m)] = s->r_vals[e];
+    }
+    std::vector<int64_t> ptr(m + 1, 0);
+    std::vector<int32_t> col;
+    std::vector<double> val;
+    for (size_t i = 0; i < m; ++i) {
+      std::sort(rows[i].begin(), rows[i].end());
+      for (auto &e : rows[i]) {
+        col.push_back(e.first);
+        val.push_back(e.second);
+      }
+      ptr[i + 1] = static_cast<int64_t>(col.size());
+    }
+    if ((st = s->d_ree_ptr.allocate(m + 1)) || (st = s->d_ree_col.allocate(std::max<size_t>(col.size(), 1))) ||
+        (st = s->d_ree_val.allocate(std::max<size_t>(val.size(), 1))) ||
+        (st = s->d_rep.allocate(std::max<size_t>(rep.size(), 1))) ||
+        (st = s->d_rpp.allocate(std::max<size_t>(rpp.size(), 1))))
+      return st;
+    FH_HIP_TRY(hipMemcpyAsync(s->d_ree_ptr.ptr, ptr.data(), ptr.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice, s->stream));
+    if (!col.empty()) {
+      FH_HIP_TRY(hipMemcpyAsync(s->d_ree_col.ptr, col.data(), col.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s->stream));
+      FH_HIP_TRY(hipMemcpyAsync(s->d_ree_val.ptr, val.data(), val.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s->stream));
+    }
+    if (!rep.empty())
+      FH_HIP_TRY(hipMemcpyAsync(s->d_rep.ptr, rep.data(), rep.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s->stream));
+    if (!rpp.empty())
+      FH_HIP_TRY(hipMemcpyAsync(s->d_rpp.ptr, rpp.data(), rpp.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s->stream));
+    FH_HIP_TRY(hipStreamSynchronize(s->stream));
+  }
   s->counts[0] = problem->n_nodes;
   s->counts[1] = problem->n_segments;
   s->counts[2] = n;
@@ -280,6 +326,8 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
   const size_t m = s->basis.internal, p = s->basis.loops - m;
   Profiler prof;
   Profiler *pp = s->profiling ? &prof : nullptr;
+  int refine_steps_max = 0;
+  double refine_change_max = 0.0;
   if (m == 0) {
     // No internal loops: Z = Zpp, nothing to solve (solve.rs:1159-1161).
     std::vector<double> lm(p * p);
@@ -295,15 +343,14 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       }
     }
   } else {
-    const size_t per = ldlt_bytes_per_frequency(m, p);
+    const bool refined = s->precision == FH_FP32_REFINED;
+    const size_t per = ldlt_bytes_per_frequency(m, p, refined);
     size_t chunk = std::max<size_t>(1, sweep_budget_bytes() / per);
     chunk = std::min(chunk, K);
     DeviceArray<double> d_omegas, d_z;
-    DeviceArray<int> d_sing;
     fh_status st;
     if ((st = d_omegas.allocate(std::max<size_t>(chunk, 1))) ||
-        (st = d_z.allocate(std::max<size_t>(chunk * p * p * 2, 1))) ||
-        (st = d_sing.allocate(std::max<size_t>(chunk, 1))))
+        (st = d_z.allocate(std::max<size_t>(chunk * p * p * 2, 1))))
       return st;
     DenseSolveInputs in;
     in.m = m;
@@ -313,24 +360,30 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     in.d_r_cols = s->d_r_cols.ptr;
     in.d_r_vals = s->d_r_vals.ptr;
     in.r_nnz = s->r_vals.size();
+    in.d_ree_ptr = s->d_ree_ptr.ptr;
+    in.d_ree_col = s->d_ree_col.ptr;
+    in.d_ree_val = s->d_ree_val.ptr;
+    in.d_rep = s->d_rep.ptr;
+    in.d_rpp = s->d_rpp.ptr;
     std::vector<double> omegas(chunk);
-    std::vector<int> sing(chunk);
     for (size_t c0 = 0; c0 < K; c0 += chunk) {
       const size_t kc = std::min(chunk, K - c0);
       for (size_t q = 0; q < kc; ++q) omegas[q] = 6.283185307179586 * freqs[c0 + q];
       FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double),
                                 hipMemcpyHostToDevice, s->stream));
-      FH_HIP_TRY(hipMemsetAsync(d_sing.ptr, 0, kc * sizeof(int), s->stream));
-      if ((st = ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, d_sing.ptr, s->ws,
-                                 s->stream, pp)))
-        return st;
+      BatchResult res;
+      st = refined ? ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
+                                        s->stream, pp)
+                   : ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
+                                      s->stream, pp);
+      if (st) return st;
       FH_HIP_TRY(hipMemcpyAsync(z_out + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s->stream));
-      FH_HIP_TRY(hipMemcpyAsync(sing.data(), d_sing.ptr, kc * sizeof(int), hipMemcpyDeviceToHost,
-                                s->stream));
       FH_HIP_TRY(hipStreamSynchronize(s->stream));
       for (size_t q = 0; q < kc; ++q) {
-        bool bad = sing[q] != 0;
+        refine_steps_max = std::max(refine_steps_max, res.refine_steps[q]);
+        refine_change_max = std::max(refine_change_max, res.refine_change[q]);
+        bool bad = res.singular[q] != 0;
         for (size_t e = 0; e < 2 * p * p && !bad; ++e)
           if (!std::isfinite(z_out[2 * (c0 + q) * p * p + e])) bad = true;
         if (bad)
@@ -338,6 +391,13 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
                            "the mesh system is singular or ill-conditioned at " +
                                std::to_string(freqs[c0 + q]) + " Hz",
                            static_cast<int64_t>(c0 + q), 0, freqs[c0 + q]);
+        if (!res.converged[q])
+          return set_error(FH_E_NOT_CONVERGED,
+                           "iterative refinement did not converge at " +
+                               std::to_string(freqs[c0 + q]) + " Hz (relative correction " +
+                               std::to_string(res.refine_change[q]) + " after " +
+                               std::to_string(res.refine_steps[q]) + " steps)",
+                           static_cast<int64_t>(c0 + q), res.refine_steps[q], freqs[c0 + q]);
       }
     }
   }
@@ -351,8 +411,8 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->inductance_s = s->inductance_s;
     info->assembly_s = s->assembly_s;
     info->solve_s = seconds_since(t0);
-    info->refine_steps_max = 0;
-    info->refine_change_max = 0.0;
+    info->refine_steps_max = refine_steps_max;
+    info->refine_change_max = refine_change_max;
     info->update_kernel_ms = pp ? prof.total_ms() : 0.0;
     info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used) : 0;
     info->update_kernel_flops = pp ? prof.flops : 0.0;

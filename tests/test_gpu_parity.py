@@ -140,3 +140,36 @@ def test_meander_2400_fp64_parity():
     errs = [rel_frobenius(z[k], z_ref[k]) for k in range(len(freqs))]
     assert max(errs) <= Z_TOL_FP64, errs
     system.close()
+
+
+Z_TOL_REFINED = 1e-10
+
+
+@pytest.mark.parametrize("name", ["busbar", "spiral", "coupled", "meander15", "cloud_manhattan"])
+def test_refined_impedance_matches_oracle(name):
+    """FP32 matrix-core factorization + FP64 refinement: Z within 1e-10."""
+    g, ports, d = _fixtures()[name]
+    packed = model.pack(g, ports, d)
+    freqs = np.concatenate([[0.0], fixtures.log_frequencies(1e3, 1e9, 11)])
+    system = MeshSystem.assemble_packed(packed, ports, "fp32_refined")
+    res = system.sweep(freqs)
+    z = res.impedance_ohm
+    z_ref = oracle.System(packed).sweep(freqs)
+    errs = [rel_frobenius(z[k], z_ref[k]) for k in range(len(freqs))]
+    assert max(errs) <= Z_TOL_REFINED, errs
+    assert np.all(z[0].imag == 0.0)
+    if system.counts().internal_meshes > 0:
+        assert res.extra["refine_steps_max"] >= 1
+    system.close()
+
+
+def test_meander_2400_refined_parity():
+    g, ports, d = fixtures.meander_problem(24)
+    packed = model.pack(g, ports, d)
+    freqs = fixtures.log_frequencies(1e4, 1e9, 21)
+    system = MeshSystem.assemble_packed(packed, ports, "fp32_refined")
+    z = system.sweep(freqs).impedance_ohm
+    z_ref = oracle.System(packed).sweep(freqs)
+    errs = [rel_frobenius(z[k], z_ref[k]) for k in range(len(freqs))]
+    assert max(errs) <= Z_TOL_REFINED, errs
+    system.close()
