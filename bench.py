@@ -105,9 +105,16 @@ def main() -> int:
     upd_ms = sum(i["update_kernel_ms"] for i in infos)
     upd_n = sum(i["update_kernel_launches"] for i in infos)
     upd_flops = sum(i["update_kernel_flops"] for i in infos)
+    upd_unit_flops = sum(i["update_kernel_unit_flops"] for i in infos)
     unit_kind = "fp64" if args.precision == "fp64" else "fp32"
-    peak = PEAKS_TFLOPS[unit_kind]
-    achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    # The roofline is priced on the units the dominant kernel runs on: FP64 or
+    # FP32 matrix cores, or the bf16 matrix cores for split-bf16 updates (three
+    # bf16 products per FP32-equivalent product).
+    arith = infos[-1]["factor_arith"]
+    peak_kind = {0: "fp64", 1: "fp32", 2: "bf16"}[arith]
+    peak = PEAKS_TFLOPS[peak_kind]
+    achieved = upd_unit_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    achieved_equiv = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
 
     extra = {}
     m = counts.internal_meshes
@@ -151,10 +158,16 @@ def main() -> int:
                                f"{m} internal loops, {args.freqs} log freqs 10 kHz-1 GHz",
                    "precision": args.precision, "filaments": counts.filaments,
                    "internal_loops": m, "frequencies": args.freqs,
-                   "assembly_s": assembly_s, "inductance_s": infos[0]["inductance_s"]},
+                   "assembly_s": assembly_s, "inductance_s": infos[0]["inductance_s"],
+                   "factor_arith": {0: "fp64", 1: "fp32-mfma", 2: "fp32 split-bf16 mfma"}[
+                       infos[-1]["factor_arith"]],
+                   "refine_steps_max": infos[-1]["refine_steps_max"],
+                   "refine_change_max": infos[-1]["refine_change_max"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "update_kernel (trailing L D L^T update)",
+                     "kernel": "outer trailing update of the L D L^T factorization "
+                               f"({peak_kind} matrix cores)",
+                     "achieved_equivalent_tflops": achieved_equiv,
                      "launches": upd_n, "avg_launch_ms": upd_ms / max(upd_n, 1)},
         "cpu_baseline": cpu,
         **extra,

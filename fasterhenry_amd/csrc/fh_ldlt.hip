@@ -43,6 +43,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fh_ldlt.hpp"
 
@@ -294,10 +295,11 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
       const int kk = kc + k;
       ps_r[k][x] = linvt_re[lbase + kk * kB + x];  // zero outside the block
       ps_i[k][x] = linvt_im[lbase + kk * kB + x];
-      const int64_t r = r0 + x;
-      const bool ok = kk < w && r < N;
-      qs_r[k][x] = ok ? Re[(k0 + kk) * ld + r] : T(0);
-      qs_i[k][x] = ok ? Im[(k0 + kk) * ld + r] : T(0);
+      const bool ok = kk < w && r0 + x < N;
+      const int64_t r = min(r0 + x, N - 1), kcl = min(kk, w - 1);
+      const T a = Re[(k0 + kcl) * ld + r], b = Im[(k0 + kcl) * ld + r];
+      qs_r[k][x] = ok ? a : T(0);
+      qs_i[k][x] = ok ? b : T(0);
     }
     __syncthreads();
     tile_mma(acc, ps_r, ps_i, qs_r, qs_i, wi, wj, lane);
@@ -362,16 +364,32 @@ __global__ void __launch_bounds__(256) update_kernel(T *__restrict__ re, T *__re
     for (int q = threadIdx.x; q < kKC * kT; q += blockDim.x) {
       const int k = q / kT, x = q % kT;
       const int kk = kc + k;
-      const int64_t s = c0 + x, r = r0 + x;
-      const bool ks = kk < kn;
-      ps_r[k][x] = (ks && s < ce) ? Re[(acol + kk) * ld + s] : T(0);
-      ps_i[k][x] = (ks && s < ce) ? Im[(acol + kk) * ld + s] : T(0);
-      qs_r[k][x] = (ks && r < N) ? Wr[(wcol + kk) * ld + r] : T(0);
-      qs_i[k][x] = (ks && r < N) ? Wi[(wcol + kk) * ld + r] : T(0);
+      const bool ks = kk < kn, sok = c0 + x < ce, rok = r0 + x < N;
+      const int64_t kcl = min(kk, kn - 1);
+      const int64_t s = min(c0 + x, ce - 1), r = min(r0 + x, N - 1);
+      const T a = Re[(acol + kcl) * ld + s], b = Im[(acol + kcl) * ld + s];
+      const T c = Wr[(wcol + kcl) * ld + r], d = Wi[(wcol + kcl) * ld + r];
+      ps_r[k][x] = (ks && sok) ? a : T(0);
+      ps_i[k][x] = (ks && sok) ? b : T(0);
+      qs_r[k][x] = (ks && rok) ? c : T(0);
+      qs_i[k][x] = (ks && rok) ? d : T(0);
     }
     __syncthreads();
     tile_mma(acc, ps_r, ps_i, qs_r, qs_i, wi, wj, lane);
   }
+  // C -= acc: all loads (clamped addresses) before the first subtraction.
+  T cr[2][2][4], ci[2][2][4];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t s = min(c0 + wi + 16 * ib + Mfma<T>::row(lane, v), ce - 1);
+        const int64_t r = min(r0 + wj + 16 * jb + (lane & 15), N - 1);
+        cr[ib][jb][v] = Re[s * ld + r];
+        ci[ib][jb][v] = Im[s * ld + r];
+      }
 #pragma unroll
   for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
@@ -381,10 +399,225 @@ __global__ void __launch_bounds__(256) update_kernel(T *__restrict__ re, T *__re
         const int64_t s = c0 + wi + 16 * ib + Mfma<T>::row(lane, v);
         const int64_t r = r0 + wj + 16 * jb + (lane & 15);
         if (s <= r && r < N && s < ce) {
-          Re[s * ld + r] -= acc.re[ib][jb][v];
-          Im[s * ld + r] -= acc.im[ib][jb][v];
+          Re[s * ld + r] = cr[ib][jb][v] - acc.re[ib][jb][v];
+          Im[s * ld + r] = ci[ib][jb][v] - acc.im[ib][jb][v];
         }
       }
+}
+
+// ---------------------------------------------------------------------------
+// 3'. The same outer update for FP32 factors on the bf16 matrix cores: each
+// FP32 operand x is split into bf16 parts x = hi + lo (+ a residual below
+// 2^-16 |x|), and each real product P Q is formed as hi hi + hi lo + lo hi
+// with FP32 accumulation (16x16x32 bf16 MFMA, 2.5 PFLOP/s dense on gfx950),
+// twelve instructions per complex 16 x 16 x 32 block.  The dropped lo lo
+// term and the split residuals bound the relative error of every product by
+// about 3 * 2^-17; refinement in FP64 removes it.  128 x 128 output tile
+// per workgroup, four waves of 64 x 64 (4 x 4 blocks); operands are split
+// once while staging into LDS as [row][k] with k contiguous, so every lane
+// reads its 8 consecutive k in one 16-byte load.
+
+// Lower-triangular tile order for the outer updates.  The T = R (R + 1) / 2
+// tiles of one frequency are listed band by band (kBand tile rows per band,
+// columns left to right, rows of the band top to bottom), and workgroup b
+// takes entry (b % 8) * ceil(T / 8) + b / 8: the hardware deals workgroups
+// round-robin to the eight XCDs, so each XCD walks one contiguous stretch of
+// the list and its L2 keeps a band's row operands and the current column
+// operand while they are reused.  Returns false for the few spare workgroups.
+constexpr int kBand = 8;
+
+__device__ __forceinline__ bool band_tile(int b, int R, int &tr, int &tc) {
+  const int T = R * (R + 1) / 2;
+  const int per = (T + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= T) return false;
+  int base = 0;
+  for (int b0 = 0; b0 < R; b0 += kBand) {
+    const int b1 = min(b0 + kBand, R);
+    // tiles in this band: sum over rows tr in [b0, b1) of (tr + 1)
+    const int n = (b1 * (b1 + 1) - b0 * (b0 + 1)) / 2;
+    if (t < base + n) {
+      int u = t - base;
+      // columns 0 .. b1 - 1; column c has rows max(b0, c) .. b1 - 1
+      for (int c = 0; c < b1; ++c) {
+        const int rows = b1 - max(b0, c);
+        if (u < rows) {
+          tc = c;
+          tr = max(b0, c) + u;
+          return true;
+        }
+        u -= rows;
+      }
+    }
+    base += n;
+  }
+  return false;
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kST = 128;     // split tile
+constexpr int kSK = 32;      // k per MFMA / per staged chunk
+constexpr int kSP = kSK + 8; // LDS row pitch (bf16), avoids bank conflicts
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+  s16x8 s = __builtin_bit_cast(s16x8, v);
+  s ^= static_cast<short>(0x8000);
+  return __builtin_bit_cast(bf16x8, s);
+}
+
+__global__ void __launch_bounds__(256) update_split_kernel(
+    float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int rs, int N,
+    int acol, int wcol, int kn, const float *__restrict__ wre, const float *__restrict__ wim,
+    int64_t w_stride) {
+  const int f = blockIdx.y;
+  int tr, tc;
+  if (!band_tile(blockIdx.x, (N - rs + kST - 1) / kST, tr, tc)) return;
+  const int r0 = rs + tr * kST, c0 = rs + tc * kST;
+  float *Re = re + f * mat_stride, *Im = im + f * mat_stride;
+  const float *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
+  // planes: 0 re hi, 1 re lo, 2 im hi, 3 im lo; [row][k], k contiguous
+  __shared__ __attribute__((aligned(16))) __bf16 ps[4][kST][kSP];
+  __shared__ __attribute__((aligned(16))) __bf16 qs[4][kST][kSP];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wi = 64 * (wave >> 1), wj = 64 * (wave & 1);
+  f4 accr[4][4], acci[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      accr[a][b] = f4{0, 0, 0, 0};
+      acci[a][b] = f4{0, 0, 0, 0};
+    }
+  const int kq = lane >> 4, x16 = lane & 15;
+  // Staging: each thread owns two (row, 8-k group) items per operand; the
+  // loads of a wave run along 64 consecutive rows of one column.
+  float pre[2][2][8], qre[2][2][8];  // [item][re/im][e]
+  // Out-of-range operands are read from a clamped in-range address and
+  // replaced by zero: every load is unconditional, so none sits behind a
+  // branch with its own wait.
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int item = threadIdx.x + 256 * u;
+      const int row = item & (kST - 1), g = item >> 7;
+      const bool sok = c0 + row < N, rok = r0 + row < N;
+      const int64_t s = min(c0 + row, N - 1), r = min(r0 + row, N - 1);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int kk = kc + 8 * g + e;
+        const bool ks = kk < kn;
+        const int64_t kcl = min(kk, kn - 1);
+        const float a = Re[(acol + kcl) * ld + s], b = Im[(acol + kcl) * ld + s];
+        const float c = Wr[(wcol + kcl) * ld + r], d = Wi[(wcol + kcl) * ld + r];
+        pre[u][0][e] = (ks && sok) ? a : 0.f;
+        pre[u][1][e] = (ks && sok) ? b : 0.f;
+        qre[u][0][e] = (ks && rok) ? c : 0.f;
+        qre[u][1][e] = (ks && rok) ? d : 0.f;
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int item = threadIdx.x + 256 * u;
+      const int row = item & (kST - 1), g = item >> 7;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        bf16x8 ph, pl, qh, ql;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          __bf16 h, l;
+          split_bf16(pre[u][c][e], h, l);
+          ph[e] = h;
+          pl[e] = l;
+          split_bf16(qre[u][c][e], h, l);
+          qh[e] = h;
+          ql[e] = l;
+        }
+        *reinterpret_cast<bf16x8 *>(&ps[2 * c][row][8 * g]) = ph;
+        *reinterpret_cast<bf16x8 *>(&ps[2 * c + 1][row][8 * g]) = pl;
+        *reinterpret_cast<bf16x8 *>(&qs[2 * c][row][8 * g]) = qh;
+        *reinterpret_cast<bf16x8 *>(&qs[2 * c + 1][row][8 * g]) = ql;
+      }
+    }
+  };
+  load(0);
+  store();
+  __syncthreads();
+  for (int kc = 0; kc < kn; kc += kSK) {
+    const bool more = kc + kSK < kn;
+    if (more) load(kc + kSK);  // in flight while the matrix cores run
+    bf16x8 p[4][4];  // [ib][plane]
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl)
+        p[ib][pl] = *reinterpret_cast<const bf16x8 *>(&ps[pl][wi + 16 * ib + x16][8 * kq]);
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) {
+      bf16x8 q[4];
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl)
+        q[pl] = *reinterpret_cast<const bf16x8 *>(&qs[pl][wj + 16 * jb + x16][8 * kq]);
+#pragma unroll
+      for (int ib = 0; ib < 4; ++ib) {
+        const bf16x8 nih = neg8(p[ib][2]), nil = neg8(p[ib][3]);
+        f4 cr = accr[ib][jb], ci = acci[ib][jb];
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[1], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[2], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[3], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, q[2], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[3], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[0], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[1], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][3], q[0], ci, 0, 0, 0);
+        accr[ib][jb] = cr;
+        acci[ib][jb] = ci;
+      }
+    }
+    __syncthreads();
+    if (more) {
+      store();
+      __syncthreads();
+    }
+  }
+  // C -= acc, one 16-row group at a time: all loads of a group are issued
+  // (from clamped addresses) before the first subtraction.
+#pragma unroll
+  for (int ib = 0; ib < 4; ++ib) {
+    float cr[4][4], ci[4][4];
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t s = min(c0 + wi + 16 * ib + 4 * kq + v, N - 1);
+        const int64_t r = min(r0 + wj + 16 * jb + x16, N - 1);
+        cr[jb][v] = Re[s * ld + r];
+        ci[jb][v] = Im[s * ld + r];
+      }
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t s = c0 + wi + 16 * ib + 4 * kq + v;
+        const int64_t r = r0 + wj + 16 * jb + x16;
+        if (s <= r && r < N) {
+          Re[s * ld + r] = cr[jb][v] - accr[ib][jb][v];
+          Im[s * ld + r] = ci[jb][v] - acci[ib][jb][v];
+        }
+      }
+  }
 }
 
 // Z_f (p x p) from the trailing block; f = 0 gets Im Z = 0 exactly
@@ -721,10 +954,25 @@ void launch_update(const Layout &L, const Factor<T> &F, int K, int rs, int cs, i
   }
 }
 
+void launch_split_update(const Layout &L, const Factor<float> &F, int K, int rs, int acol, int kn,
+                         hipStream_t stream, Profiler *prof) {
+  const int rtiles = (L.N - rs + kST - 1) / kST;
+  if (rtiles <= 0 || kn <= 0) return;
+  if (prof) prof->begin(stream);
+  const int T = rtiles * (rtiles + 1) / 2;
+  update_split_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
+      F.re, F.im, L.ld, L.mat_stride, rs, L.N, acol, 0, kn, F.wre, F.wim, L.w_stride);
+  if (prof) {
+    const double t = static_cast<double>(L.N - rs);
+    prof->end(stream, 8.0 * kn * (t * (t + 1.0) / 2.0) * K, 3.0);
+  }
+}
+
 // Factorizes (L D L^T) the first m pivots of the bordered matrices.
 template <typename T>
 fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d_omegas, int K,
-                    const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof) {
+                    const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
+                    bool split = false) {
   {
     dim3 grid(2048, K);
     form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, L.N,
@@ -755,10 +1003,19 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
           k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im);
       FH_HIP_TRY(hipGetLastError());
       if (k0 + w < KE)
-        launch_update<T>(L, F, K, k0 + w, k0 + w, KE, k0, k0 - K0, w, false, stream, prof);
+        launch_update<T>(L, F, K, k0 + w, k0 + w, KE, k0, k0 - K0, w, false, stream, nullptr);
       FH_HIP_TRY(hipGetLastError());
     }
-    if (KE < L.N) launch_update<T>(L, F, K, KE, KE, L.N, K0, 0, KE - K0, true, stream, prof);
+    if (KE < L.N) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (split) {
+          launch_split_update(L, F, K, KE, K0, KE - K0, stream, prof);
+          FH_HIP_TRY(hipGetLastError());
+          continue;
+        }
+      }
+      launch_update<T>(L, F, K, KE, KE, L.N, K0, 0, KE - K0, true, stream, prof);
+    }
     FH_HIP_TRY(hipGetLastError());
   }
   return FH_OK;
@@ -849,53 +1106,70 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   double *P = DX + static_cast<size_t>(Q) * m;
   double *metric = P + static_cast<size_t>(Q) * m;
   int *d_sing = reinterpret_cast<int *>(metric + static_cast<size_t>(K) * p * p);
-  FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
-  if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof))) return st;
   const size_t n = static_cast<size_t>(Q) * m;
   const unsigned vec_blocks = static_cast<unsigned>((n + 255) / 256);
   const unsigned res_blocks = static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256);
   const int64_t lds = L.N;
-  FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
-  FH_HIP_TRY(hipMemsetAsync(P, 0, n * sizeof(double), stream));
-  // r = b - Zee 0 = b
-  residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
-                                                  in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
-  FH_HIP_TRY(hipGetLastError());
-  out.singular.assign(K, 0);
-  out.refine_steps.assign(K, 0);
-  out.refine_change.assign(K, INFINITY);
-  out.converged.assign(K, 0);
   std::vector<double> h_metric(static_cast<size_t>(K) * p * p);
-  std::vector<double> prev(K, INFINITY);
   const double kTol = 1e-13;
-  for (int it = 1; it <= kMaxRefine; ++it) {
-    axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
-    if ((st = factor_solve<float>(L, F, V, DX, P, stream))) return st;
-    axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
-    lx_kernel<<<dim3((m + 63) / 64, (Q + 63) / 64), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X, P);
+  // Split-bf16 outer updates first; the FP32 update if that does not converge.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool split = attempt == 0;
+    out.factor_arith = split ? 2 : 1;
+    FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
+    if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof, split))) return st;
+    FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
+    FH_HIP_TRY(hipMemsetAsync(P, 0, n * sizeof(double), stream));
+    // r = b - Zee 0 = b
     residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
                                                     in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
-    functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp, in.d_lmesh,
-                                                      lds, X, R, d_z, metric);
     FH_HIP_TRY(hipGetLastError());
-    FH_HIP_TRY(hipMemcpyAsync(h_metric.data(), metric, h_metric.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, stream));
-    FH_HIP_TRY(hipStreamSynchronize(stream));
-    bool all = true;
-    for (int f = 0; f < K; ++f) {
-      if (out.converged[f]) continue;
-      double worst = 0.0;
-      for (int q = 0; q < p * p; ++q) worst = std::max(worst, h_metric[static_cast<size_t>(f) * p * p + q]);
-      if (!(worst == worst)) worst = INFINITY;
-      out.refine_steps[f] = it;
-      out.refine_change[f] = worst;
-      // Converged when the first-order correction is negligible, or when it
-      // has stopped shrinking at a level the corrected Z absorbs.
-      if (worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f]))) out.converged[f] = 1;
-      else all = false;
-      prev[f] = worst;
+    out.singular.assign(K, 0);
+    out.refine_steps.assign(K, 0);
+    out.refine_change.assign(K, INFINITY);
+    out.converged.assign(K, 0);
+    std::vector<double> prev(K, INFINITY);
+    for (int it = 1; it <= kMaxRefine; ++it) {
+      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
+      if ((st = factor_solve<float>(L, F, V, DX, P, stream))) return st;
+      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
+      lx_kernel<<<dim3((m + 63) / 64, (Q + 63) / 64), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X, P);
+      residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
+                                                      in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
+      functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
+                                                        in.d_lmesh, lds, X, R, d_z, metric);
+      FH_HIP_TRY(hipGetLastError());
+      FH_HIP_TRY(hipMemcpyAsync(h_metric.data(), metric, h_metric.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, stream));
+      FH_HIP_TRY(hipStreamSynchronize(stream));
+      bool all = true;
+      for (int f = 0; f < K; ++f) {
+        if (out.converged[f]) continue;
+        double worst = 0.0;
+        for (int q = 0; q < p * p; ++q)
+          worst = std::max(worst, h_metric[static_cast<size_t>(f) * p * p + q]);
+        if (!(worst == worst)) worst = INFINITY;
+        out.refine_steps[f] = it;
+        out.refine_change[f] = worst;
+        // Converged when the first-order correction is negligible, or when it
+        // has stopped shrinking at a level the corrected Z absorbs.
+        if (worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f]))) out.converged[f] = 1;
+        else all = false;
+        prev[f] = worst;
+      }
+      if (all) break;
+      // A contraction slower than 1/4 per step will not reach the tolerance
+      // within the budget: stop early and let the caller use more precision.
+      if (split && it >= 3) {
+        bool slow = false;
+        for (int f = 0; f < K; ++f)
+          if (!out.converged[f] && !(out.refine_change[f] < 1e-6)) slow = true;
+        if (slow) break;
+      }
     }
-    if (all) break;
+    bool ok = true;
+    for (int f = 0; f < K; ++f) ok = ok && out.converged[f];
+    if (ok) break;
   }
   FH_HIP_TRY(hipMemcpyAsync(out.singular.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost,
                             stream));

@@ -51,11 +51,13 @@ struct LdltWorkspace {
   }
 };
 
-// HIP-event timing of the dominant kernel family, on the launching stream.
+// HIP-event timing of the dominant kernel (the outer trailing update), on the
+// launching stream.  Only launches bracketed with begin/end are counted.
 struct Profiler {
   std::vector<hipEvent_t> starts, stops;
   size_t used = 0;
-  double flops = 0.0;
+  double flops = 0.0;  // algorithmic (FP32/FP64-equivalent) flops
+  double unit_flops = 0.0;  // flops on the matrix-core units used (3x for split bf16)
   ~Profiler() { reset_events(); }
   void reset_events() {
     for (auto e : starts) (void)hipEventDestroy(e);
@@ -74,10 +76,11 @@ struct Profiler {
     }
     (void)hipEventRecord(starts[used], s);
   }
-  void end(hipStream_t s, double f) {
+  void end(hipStream_t s, double f, double unit_factor = 1.0) {
     (void)hipEventRecord(stops[used], s);
     ++used;
     flops += f;
+    unit_flops += f * unit_factor;
   }
   // Sum of the recorded intervals (ms); the stream must be synchronized.
   double total_ms() const {
@@ -96,6 +99,7 @@ struct BatchResult {
   std::vector<int> refine_steps;     // per frequency (refined form)
   std::vector<double> refine_change; // per frequency: final |x^T r| / |b^T x|
   std::vector<int> converged;        // per frequency (refined form)
+  int factor_arith = 0;              // 0 FP64, 1 FP32 MFMA, 2 FP32 with split-bf16 updates
 };
 
 // Plain FP64: Z of K frequencies at once into d_z (K x p x p, {re, im}).

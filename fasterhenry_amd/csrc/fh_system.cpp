@@ -328,6 +328,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
   Profiler *pp = s->profiling ? &prof : nullptr;
   int refine_steps_max = 0;
   double refine_change_max = 0.0;
+  int factor_arith = s->precision == FH_FP32_REFINED ? 1 : 0;
   if (m == 0) {
     // No internal loops: Z = Zpp, nothing to solve (solve.rs:1159-1161).
     std::vector<double> lm(p * p);
@@ -380,6 +381,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       FH_HIP_TRY(hipMemcpyAsync(z_out + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s->stream));
       FH_HIP_TRY(hipStreamSynchronize(s->stream));
+      factor_arith = std::max(factor_arith, res.factor_arith);
       for (size_t q = 0; q < kc; ++q) {
         refine_steps_max = std::max(refine_steps_max, res.refine_steps[q]);
         refine_change_max = std::max(refine_change_max, res.refine_change[q]);
@@ -412,10 +414,12 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->assembly_s = s->assembly_s;
     info->solve_s = seconds_since(t0);
     info->refine_steps_max = refine_steps_max;
+    info->factor_arith = factor_arith;
     info->refine_change_max = refine_change_max;
     info->update_kernel_ms = pp ? prof.total_ms() : 0.0;
     info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used) : 0;
     info->update_kernel_flops = pp ? prof.flops : 0.0;
+    info->update_kernel_unit_flops = pp ? prof.unit_flops : 0.0;
   }
   return FH_OK;
 }

@@ -49,12 +49,12 @@ class SweepInfo(C.Structure):
                 ("meshes", C.c_size_t), ("internal_meshes", C.c_size_t), ("ports", C.c_size_t),
                 ("inductance_s", C.c_double), ("assembly_s", C.c_double),
                 ("solve_s", C.c_double), ("refine_steps_max", C.c_int32),
-                ("reserved", C.c_int32), ("refine_change_max", C.c_double),
+                ("factor_arith", C.c_int32), ("refine_change_max", C.c_double),
                 ("update_kernel_ms", C.c_double), ("update_kernel_launches", C.c_int64),
-                ("update_kernel_flops", C.c_double)]
+                ("update_kernel_flops", C.c_double), ("update_kernel_unit_flops", C.c_double)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None

@@ -110,13 +110,17 @@ typedef struct fh_sweep_info {
   double solve_s;      /* the sweep, from frequencies in to Z out */
   /* FH_FP32_REFINED: worst refinement steps and final relative Schur change */
   int32_t refine_steps_max;
-  int32_t reserved;
+  /* arithmetic of the trailing updates that produced the result: 0 FP64,
+   * 1 FP32 matrix cores, 2 FP32 with split-bf16 matrix-core products */
+  int32_t factor_arith;
   double refine_change_max;
   /* instrumentation (fh_system_set_profiling): time and launch count of
-   * the dominant kernel family (the trailing update), HIP events */
+   * the dominant kernel (the trailing update over each outer block), HIP
+   * events on the system's stream */
   double update_kernel_ms;
   int64_t update_kernel_launches;
-  double update_kernel_flops; /* algorithmic flops those launches did */
+  double update_kernel_flops;      /* algorithmic (FP64/FP32-equivalent) flops */
+  double update_kernel_unit_flops; /* flops executed on the matrix-core units */
 } fh_sweep_info;
 
 typedef struct fh_system fh_system;
