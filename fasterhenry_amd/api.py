@@ -171,7 +171,8 @@ class MeshSystem:
         return SweepResult(self.ports, [float(x) for x in np.asarray(frequencies, dtype=float)], z,
                            self._counts, timing,
                            extra={"refine_steps_max": info.refine_steps_max,
-                                  "refine_change_max": info.refine_change_max})
+                                  "refine_change_max": info.refine_change_max,
+                                  "factor_arith": info.factor_arith})
 
     def impedance(self, frequency: float) -> np.ndarray:
         """Z at one frequency (solve.rs:806-830)."""

@@ -39,6 +39,11 @@
 //                      costs one pass per NB columns instead of per 64.
 // The update is a 64 x 64 output tile per workgroup, four waves of 2 x 2
 // 16x16x4 MFMA blocks, complex as four real products.
+//
+// The FP32 factorization of the refined form replaces step 3's outer update
+// by a left-looking one on the bf16 matrix cores (update_left_kernel): each
+// outer block first receives the whole contribution of the columns to its
+// left in one pass, from split-bf16 copies of L and W written once per block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -406,60 +411,16 @@ __global__ void __launch_bounds__(256) update_kernel(T *__restrict__ re, T *__re
 }
 
 // ---------------------------------------------------------------------------
-// 3'. The same outer update for FP32 factors on the bf16 matrix cores: each
-// FP32 operand x is split into bf16 parts x = hi + lo (+ a residual below
+// 3'. The outer update for FP32 factors on the bf16 matrix cores: each FP32
+// operand x is split into bf16 parts x = hi + lo (+ a residual below
 // 2^-16 |x|), and each real product P Q is formed as hi hi + hi lo + lo hi
 // with FP32 accumulation (16x16x32 bf16 MFMA, 2.5 PFLOP/s dense on gfx950),
 // twelve instructions per complex 16 x 16 x 32 block.  The dropped lo lo
 // term and the split residuals bound the relative error of every product by
-// about 3 * 2^-17; refinement in FP64 removes it.  128 x 128 output tile
-// per workgroup, four waves of 64 x 64 (4 x 4 blocks); operands are split
-// once while staging into LDS as [row][k] with k contiguous, so every lane
-// reads its 8 consecutive k in one 16-byte load.
-
-// Lower-triangular tile order for the outer updates.  The T = R (R + 1) / 2
-// tiles of one frequency are listed band by band (kBand tile rows per band,
-// columns left to right, rows of the band top to bottom), and workgroup b
-// takes entry (b % 8) * ceil(T / 8) + b / 8: the hardware deals workgroups
-// round-robin to the eight XCDs, so each XCD walks one contiguous stretch of
-// the list and its L2 keeps a band's row operands and the current column
-// operand while they are reused.  Returns false for the few spare workgroups.
-constexpr int kBand = 8;
-
-__device__ __forceinline__ bool band_tile(int b, int R, int &tr, int &tc) {
-  const int T = R * (R + 1) / 2;
-  const int per = (T + 7) / 8;
-  const int t = (b % 8) * per + b / 8;
-  if (t >= T) return false;
-  int base = 0;
-  for (int b0 = 0; b0 < R; b0 += kBand) {
-    const int b1 = min(b0 + kBand, R);
-    // tiles in this band: sum over rows tr in [b0, b1) of (tr + 1)
-    const int n = (b1 * (b1 + 1) - b0 * (b0 + 1)) / 2;
-    if (t < base + n) {
-      int u = t - base;
-      // columns 0 .. b1 - 1; column c has rows max(b0, c) .. b1 - 1
-      for (int c = 0; c < b1; ++c) {
-        const int rows = b1 - max(b0, c);
-        if (u < rows) {
-          tc = c;
-          tr = max(b0, c) + u;
-          return true;
-        }
-        u -= rows;
-      }
-    }
-    base += n;
-  }
-  return false;
-}
+// about 3 * 2^-17; refinement in FP64 removes it.
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
-
-constexpr int kST = 128;     // split tile
-constexpr int kSK = 32;      // k per MFMA / per staged chunk
-constexpr int kSP = kSK + 8; // LDS row pitch (bf16), avoids bank conflicts
 
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   hi = static_cast<__bf16>(x);
@@ -472,21 +433,120 @@ __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
   return __builtin_bit_cast(bf16x8, s);
 }
 
-__global__ void __launch_bounds__(256) update_split_kernel(
-    float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int rs, int N,
-    int acol, int wcol, int kn, const float *__restrict__ wre, const float *__restrict__ wim,
-    int64_t w_stride) {
+// ---------------------------------------------------------------------------
+// 3''. Left-looking form of the split-bf16 update.  Before the
+// outer block of columns [J, J + nbj) is factorized it receives, in one
+// launch, the whole contribution of the factorized columns k < J:
+//   A[r][s] -= sum_{k < J} W[r][k] L[s][k],   r >= s, s in [J, J + nbj),
+// so every entry of A is read and written once by this kernel over the whole
+// factorization instead of once per outer block.  The operands are the split
+// copies below (written once per outer block), read straight into registers
+// with one 16-byte load per lane and fragment: no LDS, no barriers.
+//
+// Split storage.  For each 32-column chunk kc of the factor and each 16-row
+// group g >= 2 kc (the groups that meet the lower triangle of the chunk), a
+// 4 KB block of four bf16 planes (re hi, re lo, im hi, im lo), each 64 lanes
+// x 8 values in the operand order of the 16x16x32 bf16 MFMA: lane l holds
+// row 16 g + l % 16, columns 32 kc + 8 (l / 16) + e.  Block index
+// kc G - kc (kc - 1) + (g - 2 kc), G = ceil(N / 16).
+
+__host__ __device__ __forceinline__ int64_t split_block(int kc, int g, int G) {
+  return static_cast<int64_t>(kc) * G - static_cast<int64_t>(kc) * (kc - 1) + (g - 2 * kc);
+}
+
+// Split copies of L (lsplit) and W = L D (wsplit) for the chunks
+// [kc0, kc0 + kcn) of the outer block at J and the row groups [g0, G);
+// rows >= N are zero.  One wave per (chunk, group) block.
+__global__ void __launch_bounds__(256) split_panel_kernel(
+    const float *__restrict__ re, const float *__restrict__ im, int64_t ld, int64_t mat_stride,
+    const float *__restrict__ wre, const float *__restrict__ wim, int64_t w_stride, int J,
+    int kc0, int kcn, int g0, int gn, int N, int G, bf16x8 *__restrict__ lsplit,
+    bf16x8 *__restrict__ wsplit, int64_t split_stride) {
+  const int f = blockIdx.y;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int item = blockIdx.x * 4 + wave;  // (chunk, group) pair
+  if (item >= kcn * gn) return;
+  const int kc = kc0 + item / gn, g = g0 + item % gn;
+  const float *Re = re + f * mat_stride, *Im = im + f * mat_stride;
+  const float *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
+  const int row = 16 * g + (lane & 15);
+  const bool ok = row < N;
+  const int64_t rcl = min(row, N - 1);
+  const int kbase = 32 * kc + 8 * (lane >> 4);
+  bf16x8 lp[4], wp[4];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = kbase + e;
+    const float a = Re[k * ld + rcl], b = Im[k * ld + rcl];
+    const float c = Wr[(k - J) * ld + rcl], d = Wi[(k - J) * ld + rcl];
+    __bf16 h, l;
+    split_bf16(ok ? a : 0.f, h, l);
+    lp[0][e] = h;
+    lp[1][e] = l;
+    split_bf16(ok ? b : 0.f, h, l);
+    lp[2][e] = h;
+    lp[3][e] = l;
+    split_bf16(ok ? c : 0.f, h, l);
+    wp[0][e] = h;
+    wp[1][e] = l;
+    split_bf16(ok ? d : 0.f, h, l);
+    wp[2][e] = h;
+    wp[3][e] = l;
+  }
+  const int64_t base = f * split_stride + split_block(kc, g, G) * 4 * 64 + lane;
+#pragma unroll
+  for (int pl = 0; pl < 4; ++pl) {
+    lsplit[base + pl * 64] = lp[pl];
+    wsplit[base + pl * 64] = wp[pl];
+  }
+}
+
+// Tiles of the left-looking update: 128 x 128, rows r in [J, N), columns s
+// in [J, J + nbj); tile (tr, tc) with tc < ctiles and tr >= tc.  Listed row
+// by row, so the tiles of one row (sharing the W operand) are adjacent, and
+// dealt to the XCDs in contiguous stretches (the hardware assigns
+// workgroup b to XCD b % 8).
+__device__ __forceinline__ bool left_tile(int b, int rtiles, int ctiles, int &tr, int &tc) {
+  const int tri = ctiles * (ctiles - 1) / 2;
+  const int T = tri + (rtiles - (ctiles - 1)) * ctiles;
+  const int per = (T + 7) / 8;
+  const int t = (b % 8) * per + b / 8;
+  if (t >= T) return false;
+  if (t < tri) {
+    tr = static_cast<int>((__builtin_sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (tr * (tr + 1) / 2 > t) --tr;
+    while ((tr + 1) * (tr + 2) / 2 <= t) ++tr;
+    tc = t - tr * (tr + 1) / 2;
+  } else {
+    const int u = t - tri;
+    tr = ctiles - 1 + u / ctiles;
+    tc = u % ctiles;
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(256) update_left_kernel(
+    float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int J,
+    int nbj, int N, int G, int rtiles, int ctiles, const bf16x8 *__restrict__ lsplit,
+    const bf16x8 *__restrict__ wsplit, int64_t split_stride) {
   const int f = blockIdx.y;
   int tr, tc;
-  if (!band_tile(blockIdx.x, (N - rs + kST - 1) / kST, tr, tc)) return;
-  const int r0 = rs + tr * kST, c0 = rs + tc * kST;
-  float *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  const float *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
-  // planes: 0 re hi, 1 re lo, 2 im hi, 3 im lo; [row][k], k contiguous
-  __shared__ __attribute__((aligned(16))) __bf16 ps[4][kST][kSP];
-  __shared__ __attribute__((aligned(16))) __bf16 qs[4][kST][kSP];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (!left_tile(blockIdx.x, rtiles, ctiles, tr, tc)) return;
+  // The wave index is uniform across the wave: keeping it (and every
+  // address derived from it) in scalar registers leaves the vector
+  // registers to the operands and accumulators.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int wi = 64 * (wave >> 1), wj = 64 * (wave & 1);
+  const int s0 = J + 128 * tc + wi, r0 = J + 128 * tr + wj;
+  const bf16x8 *Ls = lsplit + f * split_stride;
+  const bf16x8 *Ws = wsplit + f * split_stride;
+  int gs[4], gr[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    gs[b] = min(s0 / 16 + b, G - 1);
+    gr[b] = min(r0 / 16 + b, G - 1);
+  }
   f4 accr[4][4], acci[4][4];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -495,105 +555,68 @@ __global__ void __launch_bounds__(256) update_split_kernel(
       accr[a][b] = f4{0, 0, 0, 0};
       acci[a][b] = f4{0, 0, 0, 0};
     }
-  const int kq = lane >> 4, x16 = lane & 15;
-  // Staging: each thread owns two (row, 8-k group) items per operand; the
-  // loads of a wave run along 64 consecutive rows of one column.
-  float pre[2][2][8], qre[2][2][8];  // [item][re/im][e]
-  // Out-of-range operands are read from a clamped in-range address and
-  // replaced by zero: every load is unconditional, so none sits behind a
-  // branch with its own wait.
-  auto load = [&](int kc) {
+  bf16x8 p[4][4], qa[4][4], qb[4][4];
+  auto load_p = [&](int kc, int b) {
+    const bf16x8 *bp = Ls + split_block(kc, gs[b], G) * 256;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int item = threadIdx.x + 256 * u;
-      const int row = item & (kST - 1), g = item >> 7;
-      const bool sok = c0 + row < N, rok = r0 + row < N;
-      const int64_t s = min(c0 + row, N - 1), r = min(r0 + row, N - 1);
+    for (int pl = 0; pl < 4; ++pl) p[b][pl] = bp[64 * pl + lane];
+  };
+  auto load_q = [&](int kc, bf16x8 (&q)[4][4]) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int kk = kc + 8 * g + e;
-        const bool ks = kk < kn;
-        const int64_t kcl = min(kk, kn - 1);
-        const float a = Re[(acol + kcl) * ld + s], b = Im[(acol + kcl) * ld + s];
-        const float c = Wr[(wcol + kcl) * ld + r], d = Wi[(wcol + kcl) * ld + r];
-        pre[u][0][e] = (ks && sok) ? a : 0.f;
-        pre[u][1][e] = (ks && sok) ? b : 0.f;
-        qre[u][0][e] = (ks && rok) ? c : 0.f;
-        qre[u][1][e] = (ks && rok) ? d : 0.f;
-      }
+    for (int b = 0; b < 4; ++b) {
+      const bf16x8 *bq = Ws + split_block(kc, gr[b], G) * 256;
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl) q[b][pl] = bq[64 * pl + lane];
     }
   };
-  auto store = [&]() {
+  const int nkc = J / 32;
+  // One chunk of k: the 16-row group ib of the L operand is refilled with
+  // chunk kc + 1 as soon as its products are issued, so the loads of the
+  // next chunk are in flight behind the remaining matrix-core work (the W
+  // operand is double-buffered; loads past the last chunk re-read it).
+  auto step = [&](int kc, const bf16x8 (&q)[4][4]) {
+    const int kn = min(kc + 1, nkc - 1);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int item = threadIdx.x + 256 * u;
-      const int row = item & (kST - 1), g = item >> 7;
+    for (int ib = 0; ib < 4; ++ib) {
+      const bf16x8 nih = neg8(p[ib][2]), nil = neg8(p[ib][3]);
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        bf16x8 ph, pl, qh, ql;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          __bf16 h, l;
-          split_bf16(pre[u][c][e], h, l);
-          ph[e] = h;
-          pl[e] = l;
-          split_bf16(qre[u][c][e], h, l);
-          qh[e] = h;
-          ql[e] = l;
-        }
-        *reinterpret_cast<bf16x8 *>(&ps[2 * c][row][8 * g]) = ph;
-        *reinterpret_cast<bf16x8 *>(&ps[2 * c + 1][row][8 * g]) = pl;
-        *reinterpret_cast<bf16x8 *>(&qs[2 * c][row][8 * g]) = qh;
-        *reinterpret_cast<bf16x8 *>(&qs[2 * c + 1][row][8 * g]) = ql;
-      }
-    }
-  };
-  load(0);
-  store();
-  __syncthreads();
-  for (int kc = 0; kc < kn; kc += kSK) {
-    const bool more = kc + kSK < kn;
-    if (more) load(kc + kSK);  // in flight while the matrix cores run
-    bf16x8 p[4][4];  // [ib][plane]
-#pragma unroll
-    for (int ib = 0; ib < 4; ++ib)
-#pragma unroll
-      for (int pl = 0; pl < 4; ++pl)
-        p[ib][pl] = *reinterpret_cast<const bf16x8 *>(&ps[pl][wi + 16 * ib + x16][8 * kq]);
-#pragma unroll
-    for (int jb = 0; jb < 4; ++jb) {
-      bf16x8 q[4];
-#pragma unroll
-      for (int pl = 0; pl < 4; ++pl)
-        q[pl] = *reinterpret_cast<const bf16x8 *>(&qs[pl][wj + 16 * jb + x16][8 * kq]);
-#pragma unroll
-      for (int ib = 0; ib < 4; ++ib) {
-        const bf16x8 nih = neg8(p[ib][2]), nil = neg8(p[ib][3]);
+      for (int jb = 0; jb < 4; ++jb) {
         f4 cr = accr[ib][jb], ci = acci[ib][jb];
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[0], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[1], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[0], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[2], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[3], cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, q[2], cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[2], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[3], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[2], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[0], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[1], ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][3], q[0], ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][1], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[jb][0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][2], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][3], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, q[jb][2], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][3], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[jb][2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[jb][0], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[jb][1], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][3], q[jb][0], ci, 0, 0, 0);
         accr[ib][jb] = cr;
         acci[ib][jb] = ci;
       }
+      load_p(kn, ib);
+      // keep the refill here, ahead of the next group's products
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __syncthreads();
-    if (more) {
-      store();
-      __syncthreads();
-    }
+  };
+#pragma unroll
+  for (int b = 0; b < 4; ++b) load_p(0, b);
+  load_q(0, qa);
+  // nkc is even (J is a multiple of 64): the loop body has no branch for
+  // the compiler to sink the refills into.
+  for (int kc = 0; kc < nkc; kc += 2) {
+    load_q(kc + 1, qb);
+    __builtin_amdgcn_sched_barrier(0);
+    step(kc, qa);
+    load_q(min(kc + 2, nkc - 1), qa);
+    __builtin_amdgcn_sched_barrier(0);
+    step(kc + 1, qb);
   }
-  // C -= acc, one 16-row group at a time: all loads of a group are issued
-  // (from clamped addresses) before the first subtraction.
+  const int kq = lane >> 4, x16 = lane & 15;
+  const int se = J + nbj;
 #pragma unroll
   for (int ib = 0; ib < 4; ++ib) {
     float cr[4][4], ci[4][4];
@@ -601,20 +624,20 @@ __global__ void __launch_bounds__(256) update_split_kernel(
     for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t s = min(c0 + wi + 16 * ib + 4 * kq + v, N - 1);
-        const int64_t r = min(r0 + wj + 16 * jb + x16, N - 1);
-        cr[jb][v] = Re[s * ld + r];
-        ci[jb][v] = Im[s * ld + r];
+        const int64_t s = min(s0 + 16 * ib + 4 * kq + v, N - 1);
+        const int64_t r = min(r0 + 16 * jb + x16, N - 1);
+        cr[jb][v] = re[f * mat_stride + s * ld + r];
+        ci[jb][v] = im[f * mat_stride + s * ld + r];
       }
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t s = c0 + wi + 16 * ib + 4 * kq + v;
-        const int64_t r = r0 + wj + 16 * jb + x16;
-        if (s <= r && r < N) {
-          Re[s * ld + r] = cr[jb][v] - accr[ib][jb][v];
-          Im[s * ld + r] = ci[jb][v] - acci[ib][jb][v];
+        const int64_t s = s0 + 16 * ib + 4 * kq + v;
+        const int64_t r = r0 + 16 * jb + x16;
+        if (s <= r && r < N && s < se) {
+          re[f * mat_stride + s * ld + r] = cr[jb][v] - accr[ib][jb][v];
+          im[f * mat_stride + s * ld + r] = ci[jb][v] - acci[ib][jb][v];
         }
       }
   }
@@ -954,25 +977,53 @@ void launch_update(const Layout &L, const Factor<T> &F, int K, int rs, int cs, i
   }
 }
 
-void launch_split_update(const Layout &L, const Factor<float> &F, int K, int rs, int acol, int kn,
-                         hipStream_t stream, Profiler *prof) {
-  const int rtiles = (L.N - rs + kST - 1) / kST;
-  if (rtiles <= 0 || kn <= 0) return;
+// Split operand copies of the left-looking form, per frequency.
+struct SplitBufs {
+  bf16x8 *l = nullptr, *w = nullptr;
+  int G = 0;
+  int64_t stride = 0;  // bf16x8 elements per frequency
+};
+
+int64_t split_blocks(const Layout &L) {
+  const int G = (L.N + 15) / 16, KC = (L.m + 31) / 32;
+  return split_block(KC, 2 * KC, G);
+}
+
+void launch_left_update(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K, int J,
+                        int nbj, hipStream_t stream, Profiler *prof) {
+  // the kernel takes k in pairs of 32-column chunks
+  if (J % 64 != 0) return;
+  const int rtiles = (L.N - J + 127) / 128, ctiles = (nbj + 127) / 128;
+  const int T = ctiles * (ctiles - 1) / 2 + (rtiles - (ctiles - 1)) * ctiles;
   if (prof) prof->begin(stream);
-  const int T = rtiles * (rtiles + 1) / 2;
-  update_split_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
-      F.re, F.im, L.ld, L.mat_stride, rs, L.N, acol, 0, kn, F.wre, F.wim, L.w_stride);
+  update_left_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
+      F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, sb.l, sb.w, sb.stride);
   if (prof) {
-    const double t = static_cast<double>(L.N - rs);
-    prof->end(stream, 8.0 * kn * (t * (t + 1.0) / 2.0) * K, 3.0);
+    double entries = 0.0;
+    for (int s = J; s < J + nbj; ++s) entries += static_cast<double>(L.N - s);
+    prof->end(stream, 8.0 * J * entries * K, 3.0);
   }
 }
 
-// Factorizes (L D L^T) the first m pivots of the bordered matrices.
+void launch_split_panel(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K, int J,
+                        int nbj, hipStream_t stream) {
+  const int kc0 = J / 32, kcn = nbj / 32;
+  const int g0 = (J + nbj) / 16, gn = sb.G - g0;
+  if (kcn <= 0 || gn <= 0) return;
+  const int items = kcn * gn;
+  split_panel_kernel<<<dim3((items + 3) / 4, K), 256, 0, stream>>>(
+      F.re, F.im, L.ld, L.mat_stride, F.wre, F.wim, L.w_stride, J, kc0, kcn, g0, gn, L.N, sb.G,
+      sb.l, sb.w, sb.stride);
+}
+
+// Factorizes (L D L^T) the first m pivots of the bordered matrices.  With
+// `sb` (FP32 only) the outer updates are left-looking on the split-bf16
+// copies; the trailing p x p block is then not updated (the refined form
+// takes Z from the refinement, not from the factor).
 template <typename T>
 fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d_omegas, int K,
                     const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
-                    bool split = false) {
+                    const SplitBufs *sb = nullptr) {
   {
     dim3 grid(2048, K);
     form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, L.N,
@@ -989,6 +1040,12 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
   }
   for (int K0 = 0; K0 < L.m; K0 += L.nb) {
     const int KE = std::min(K0 + L.nb, L.m);
+    if constexpr (std::is_same<T, float>::value) {
+      if (sb && K0 > 0) {
+        launch_left_update(L, F, *sb, K, K0, KE - K0, stream, prof);
+        FH_HIP_TRY(hipGetLastError());
+      }
+    }
     for (int k0 = K0; k0 < KE; k0 += kB) {
       const int w = std::min(kB, KE - k0);
       const int blk = k0 / kB;
@@ -1008,8 +1065,8 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
     }
     if (KE < L.N) {
       if constexpr (std::is_same<T, float>::value) {
-        if (split) {
-          launch_split_update(L, F, K, KE, K0, KE - K0, stream, prof);
+        if (sb) {
+          if (KE < L.m) launch_split_panel(L, F, *sb, K, K0, KE - K0, stream);
           FH_HIP_TRY(hipGetLastError());
           continue;
         }
@@ -1059,8 +1116,9 @@ size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined) {
   in.m = m;
   in.p = p;
   if (!refined) return factor_bytes<double>(layout_of(in, kNbPlain), 1);
-  return factor_bytes<float>(layout_of(in, kNbRefined), 1) +
-         static_cast<size_t>(p) * m * 2 * 4 * sizeof(double);
+  const Layout L = layout_of(in, kNbRefined);
+  return factor_bytes<float>(L, 1) + static_cast<size_t>(p) * m * 2 * 4 * sizeof(double) +
+         2 * static_cast<size_t>(split_blocks(L)) * 4096;
 }
 
 fh_status ldlt_solve_batch(const DenseSolveInputs &in, const double *d_omegas, int K, double *d_z,
@@ -1097,8 +1155,17 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   const size_t vbytes = static_cast<size_t>(Q) * m * sizeof(double);
   const size_t extra = 4 * vbytes + static_cast<size_t>(K) * p * p * sizeof(double) +
                        static_cast<size_t>(K) * sizeof(int) + 256;
-  if ((st = ws.reserve(fbytes + extra))) return st;
+  const size_t sbytes = static_cast<size_t>(split_blocks(L)) * 4096 * K;
+  if ((st = ws.reserve(fbytes + extra + 2 * sbytes + 512))) return st;
   const Factor<float> F = carve<float>(ws.base, L, K);
+  SplitBufs sb;
+  sb.G = (L.N + 15) / 16;
+  sb.stride = split_blocks(L) * 256;
+  {
+    char *sp = ws.base + ((fbytes + extra + 255) / 256) * 256;
+    sb.l = reinterpret_cast<bf16x8 *>(sp);
+    sb.w = reinterpret_cast<bf16x8 *>(sp + sbytes);
+  }
   char *cur = ws.base + ((fbytes + 255) / 256) * 256;
   double *X = reinterpret_cast<double *>(cur);
   double *R = X + static_cast<size_t>(Q) * m;
@@ -1117,7 +1184,9 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     const bool split = attempt == 0;
     out.factor_arith = split ? 2 : 1;
     FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
-    if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof, split))) return st;
+    if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof,
+                               split ? &sb : nullptr)))
+      return st;
     FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
     FH_HIP_TRY(hipMemsetAsync(P, 0, n * sizeof(double), stream));
     // r = b - Zee 0 = b
