@@ -99,22 +99,23 @@ __device__ __forceinline__ Z2 zrecip(Z2 d) {
 }
 
 // ---------------------------------------------------------------------------
-// Forming Z(w) = R + j w L on the lower triangle (solve.rs:1003-1009).  l_mesh
-// is symmetric, so its row c is read for column c of A (row-contiguous).
+// Forming Z(w) = R + j w L on the lower triangle (solve.rs:1003-1009), all
+// frequencies of the batch per thread: l_mesh is read once.  l_mesh is
+// symmetric, so its row c is read for column c of A (row-contiguous).  Only
+// the lower triangle (r >= c, r < N) is written; no kernel reads the rest.
 
 template <typename T>
 __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
-                            const double *__restrict__ omegas, int N, int64_t ld,
+                            const double *__restrict__ omegas, int K, int N, int64_t ld,
                             T *__restrict__ re, T *__restrict__ im, int64_t mat_stride) {
-  const int f = blockIdx.y;
-  const double omega = omegas[f];
-  T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  const int64_t total = static_cast<int64_t>(N) * ld;
-  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
-       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int64_t c = idx / ld, r = idx % ld;
-    Re[idx] = T(0);
-    Im[idx] = (r >= N || r < c) ? T(0) : static_cast<T>(omega * lmesh[c * loops + r]);
+  const int c = blockIdx.y;
+  const int r = c + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const double l = lmesh[c * loops + r];
+  const int64_t a = c * ld + r;
+  for (int f = 0; f < K; ++f) {
+    re[f * mat_stride + a] = T(0);
+    im[f * mat_stride + a] = static_cast<T>(omegas[f] * l);
   }
 }
 
@@ -134,6 +135,21 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // 1. Diagonal block at k0 (width w <= 64, block index blk): A11 = L11 D L11^T,
 // then L11^-1.  Writes L11 and D in place, linvt[k][c] = Linv[c][k] (zero
 // outside the block) and D^-1 for block blk of frequency f.
+//
+// One workgroup per frequency; lane r of every wave holds row r, and wave g
+// holds the columns j = g (mod 4) in registers (16 slots), so a column step
+// costs each lane 16 complex multiply-adds and one barrier.  Pivot columns
+// reach the other waves through a two-entry LDS ring.  Linv is built in the
+// same pass: the row operations of the elimination (row r -= l_rc row c) are
+// applied to an identity X, which becomes L11^-1; X[c][j] is read from lane
+// c of the owning wave.
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffu), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__restrict__ im,
@@ -144,77 +160,98 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
                                                    int *__restrict__ singular) {
   const int f = blockIdx.x;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  __shared__ double sr[kB][kB + 1], si[kB][kB + 1];
-  __shared__ double lr[kB][kB + 1], li[kB][kB + 1];  // Linv[r][c]
-  for (int q = threadIdx.x; q < w * w; q += blockDim.x) {
-    const int c = q / w, r = q % w;
-    if (r >= c) {
-      sr[r][c] = static_cast<double>(Re[(k0 + c) * ld + k0 + r]);
-      si[r][c] = static_cast<double>(Im[(k0 + c) * ld + k0 + r]);
-    }
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = threadIdx.x & 63;
+  const int rc = min(r, w - 1);
+  __shared__ double cbr[2][kB], cbi[2][kB];
+  // L11 by columns, D and D^-1: written to memory after the loop, so that
+  // no global store is outstanding at the barriers of the column steps
+  __shared__ double lsr[kB][kB], lsi[kB][kB], dsr[kB], dsi[kB], vsr[kB], vsi[kB];
+  double ar[16], ai[16], xr[16], xi[16];
+#pragma unroll
+  for (int js = 0; js < 16; ++js) {
+    const int j = 4 * js + g;
+    const bool in = r < w && j <= r;
+    const int64_t a = static_cast<int64_t>(k0 + min(j, rc)) * ld + k0 + rc;
+    const double vr = static_cast<double>(Re[a]), vi = static_cast<double>(Im[a]);
+    ar[js] = in ? vr : 0.0;
+    ai[js] = in ? vi : 0.0;
+    xr[js] = (j == r) ? 1.0 : 0.0;
+    xi[js] = 0.0;
   }
-  __syncthreads();
-  for (int c = 0; c < w; ++c) {
-    const Z2 d{sr[c][c], si[c][c]};
-    if (d.re == 0.0 && d.im == 0.0 && threadIdx.x == 0) singular[f] = 1;
-    const Z2 inv = zrecip(d);
-    const int rem = w - c - 1;
-    for (int q = threadIdx.x; q < rem * rem; q += blockDim.x) {
-      const int r = c + 1 + q / rem, j = c + 1 + q % rem;
-      if (j > r) continue;
-      const Z2 l = zmul(Z2{sr[r][c], si[r][c]}, inv);
-      const Z2 p = zmul(l, Z2{sr[j][c], si[j][c]});
-      sr[r][j] -= p.re;
-      si[r][j] -= p.im;
-    }
-    __syncthreads();
-    for (int r = c + 1 + threadIdx.x; r < w; r += blockDim.x) {
-      const Z2 l = zmul(Z2{sr[r][c], si[r][c]}, inv);
-      sr[r][c] = l.re;
-      si[r][c] = l.im;
-    }
-    __syncthreads();
-  }
-  // Linv row by row: Linv[r][c] = -(L[r][c] + sum_{c<j<r} L[r][j] Linv[j][c]).
-  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
-    const int r = q / kB, c = q % kB;
-    lr[r][c] = (r == c) ? 1.0 : 0.0;
-    li[r][c] = 0.0;
-  }
-  __syncthreads();
-  for (int r = 1; r < w; ++r) {
-    for (int c = threadIdx.x; c < r; c += blockDim.x) {
-      Z2 acc{sr[r][c], si[r][c]};
-      for (int j = c + 1; j < r; ++j) {
-        const Z2 p = zmul(Z2{sr[r][j], si[r][j]}, Z2{lr[j][c], li[j][c]});
-        acc.re += p.re;
-        acc.im += p.im;
-      }
-      lr[r][c] = -acc.re;
-      li[r][c] = -acc.im;
-    }
-    __syncthreads();
-  }
-  for (int q = threadIdx.x; q < w * w; q += blockDim.x) {
-    const int c = q / w, r = q % w;
-    if (r >= c) {
-      Re[(k0 + c) * ld + k0 + r] = static_cast<T>(sr[r][c]);
-      Im[(k0 + c) * ld + k0 + r] = static_cast<T>(si[r][c]);
-    }
-  }
-  const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
-  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
-    const int k = q / kB, c = q % kB;
-    const bool in = k < w && c < w;
-    linvt_re[lbase + q] = in ? static_cast<T>(lr[c][k]) : T(0);
-    linvt_im[lbase + q] = in ? static_cast<T>(li[c][k]) : T(0);
+  if (g == 0) {
+    cbr[0][r] = ar[0];
+    cbi[0][r] = ai[0];
   }
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
-  for (int c = threadIdx.x; c < kB; c += blockDim.x) {
-    Z2 inv{0.0, 0.0};
-    if (c < w) inv = zrecip(Z2{sr[c][c], si[c][c]});
-    dinv_re[dbase + c] = static_cast<T>(inv.re);
-    dinv_im[dbase + c] = static_cast<T>(inv.im);
+  __syncthreads();
+  for (int c = 0; c < w; ++c) {
+    const int b = c & 1;
+    const Z2 d{cbr[b][c], cbi[b][c]};
+    const Z2 inv = zrecip(d);
+    const bool below = r > c && r < w;
+    const Z2 lv = zmul(Z2{cbr[b][r], cbi[b][r]}, inv);
+    const Z2 l{below ? lv.re : 0.0, below ? lv.im : 0.0};
+    if (g == 0) {
+      lsr[c][r] = l.re;
+      lsi[c][r] = l.im;
+      if (r == 0) {
+        dsr[c] = d.re;
+        dsi[c] = d.im;
+        vsr[c] = inv.re;
+        vsi[c] = inv.im;
+      }
+    }
+    // Slot js, column j: A[r][j] -= l_r A[j][c] for j > c, else
+    // X[r][j] -= l_r X[c][j].  Written without branches (one product, the
+    // other array gets -0): a branch here makes the compiler copy the
+    // register arrays through every arm.
+    // all reads of the ring entry before its other entry is written
+    double bcr[16], bci[16];
+#pragma unroll
+    for (int js = 0; js < 16; ++js) {
+      bcr[js] = cbr[b][4 * js + g];
+      bci[js] = cbi[b][4 * js + g];
+    }
+#pragma unroll
+    for (int js = 0; js < 16; ++js) {
+      const int j = 4 * js + g;
+      const bool upd_a = j > c;
+      const double br = bcr[js], bi = bci[js];
+      const double xcr = readlane_f64(xr[js], c), xci = readlane_f64(xi[js], c);
+      const Z2 pr = zmul(l, Z2{upd_a ? br : xcr, upd_a ? bi : xci});
+      ar[js] -= upd_a ? pr.re : 0.0;
+      ai[js] -= upd_a ? pr.im : 0.0;
+      xr[js] -= upd_a ? 0.0 : pr.re;
+      xi[js] -= upd_a ? 0.0 : pr.im;
+      if (j == c + 1) {  // column c + 1 goes to the other ring entry
+        cbr[b ^ 1][r] = ar[js];
+        cbi[b ^ 1][r] = ai[js];
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
+#pragma unroll
+  for (int js = 0; js < 16; ++js) {
+    const int j = 4 * js + g;
+    const bool in = r < w && j < w;
+    linvt_re[lbase + j * kB + r] = in ? static_cast<T>(xr[js]) : T(0);
+    linvt_im[lbase + j * kB + r] = in ? static_cast<T>(xi[js]) : T(0);
+  }
+  // L11 and D in place (column c, rows c..w-1), D^-1, the zero-pivot flag
+  for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
+    const int c = q >> 6, rr = q & 63;
+    if (c < w && rr >= c && rr < w) {
+      const int64_t a = static_cast<int64_t>(k0 + c) * ld + k0 + rr;
+      Re[a] = static_cast<T>(rr == c ? dsr[c] : lsr[c][rr]);
+      Im[a] = static_cast<T>(rr == c ? dsi[c] : lsi[c][rr]);
+    }
+  }
+  if (g == 0) {
+    dinv_re[dbase + r] = r < w ? static_cast<T>(vsr[r]) : T(0);
+    dinv_im[dbase + r] = r < w ? static_cast<T>(vsi[r]) : T(0);
+    if (r < w && dsr[r] == 0.0 && dsi[r] == 0.0) singular[f] = 1;
   }
 }
 
@@ -435,9 +472,9 @@ __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
 
 // ---------------------------------------------------------------------------
 // 3''. Left-looking form of the split-bf16 update.  Before the
-// outer block of columns [J, J + nbj) is factorized it receives, in one
-// launch, the whole contribution of the factorized columns k < J:
-//   A[r][s] -= sum_{k < J} W[r][k] L[s][k],   r >= s, s in [J, J + nbj),
+// columns [J, J + nbj) are factorized they receive, in one launch, the
+// contribution of the factorized columns k in [32 kc_beg, 32 kc_end):
+//   A[r][s] -= sum_k W[r][k] L[s][k],   r >= s, s in [J, J + nbj),
 // so every entry of A is read and written once by this kernel over the whole
 // factorization instead of once per outer block.  The operands are the split
 // copies below (written once per outer block), read straight into registers
@@ -527,8 +564,8 @@ __device__ __forceinline__ bool left_tile(int b, int rtiles, int ctiles, int &tr
 
 __global__ void __launch_bounds__(256) update_left_kernel(
     float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int J,
-    int nbj, int N, int G, int rtiles, int ctiles, const bf16x8 *__restrict__ lsplit,
-    const bf16x8 *__restrict__ wsplit, int64_t split_stride) {
+    int nbj, int N, int G, int rtiles, int ctiles, int kc_beg, int kc_end,
+    const bf16x8 *__restrict__ lsplit, const bf16x8 *__restrict__ wsplit, int64_t split_stride) {
   const int f = blockIdx.y;
   int tr, tc;
   if (!left_tile(blockIdx.x, rtiles, ctiles, tr, tc)) return;
@@ -569,7 +606,7 @@ __global__ void __launch_bounds__(256) update_left_kernel(
       for (int pl = 0; pl < 4; ++pl) q[b][pl] = bq[64 * pl + lane];
     }
   };
-  const int nkc = J / 32;
+  const int nkc = kc_end;
   // One chunk of k: the 16-row group ib of the L operand is refilled with
   // chunk kc + 1 as soon as its products are issued, so the loads of the
   // next chunk are in flight behind the remaining matrix-core work (the W
@@ -603,11 +640,11 @@ __global__ void __launch_bounds__(256) update_left_kernel(
     }
   };
 #pragma unroll
-  for (int b = 0; b < 4; ++b) load_p(0, b);
-  load_q(0, qa);
-  // nkc is even (J is a multiple of 64): the loop body has no branch for
-  // the compiler to sink the refills into.
-  for (int kc = 0; kc < nkc; kc += 2) {
+  for (int b = 0; b < 4; ++b) load_p(kc_beg, b);
+  load_q(kc_beg, qa);
+  // kc_end - kc_beg is even: the loop body has no branch for the compiler
+  // to sink the refills into.
+  for (int kc = kc_beg; kc < nkc; kc += 2) {
     load_q(kc + 1, qb);
     __builtin_amdgcn_sched_barrier(0);
     step(kc, qa);
@@ -663,63 +700,105 @@ __global__ void extract_kernel(const T *__restrict__ re, const T *__restrict__ i
 // ---------------------------------------------------------------------------
 // Refinement.  Vectors are FP64, [2 v][m] for re and [2 v + 1][m] for im,
 // v = f p + q (frequency f, port column q).  Solves with the stored factors
-// run one launch per 64-row block over all vectors: the block's diagonal
-// solve is a product with the stored inverse (computed redundantly by every
-// workgroup of the launch), and the rest of the launch updates the other
-// rows, each thread one row (forward: rows below, reading the block's
-// columns row-contiguously) or one column (backward: columns above, reading
-// its 64 entries in the block's rows contiguously).
+// go by super-blocks of 256 rows, two launches each over all vectors: the
+// diagonal part (one workgroup per vector; four 64-row sub-blocks, each a
+// product with the stored inverse of its unit-lower block and an update of
+// the sub-blocks after it), then the off-diagonal update of the other rows
+// (forward: the rows below, one row per thread, the block's columns read
+// along rows; backward: the rows above, through 64 x 64 tiles of L staged in
+// LDS so that the transposed product reads L along its columns).
+constexpr int kSB = 256;
 
-// Forward, block blk: y_b := Linv_b y_b written to out; rows below:
-// y[r] -= sum_c L[r][b0 + c] y_b[c].
+// Forward, diagonal part of the super-block at B0 (bw rows): solves
+// L_BB z = y_B for y_B = work[B0, B0 + bw) and writes z to out.
 template <typename T>
-__global__ void __launch_bounds__(256) fwd_block_kernel(
+__global__ void __launch_bounds__(256) fwd_diag_kernel(
     const T *__restrict__ re, const T *__restrict__ im, int64_t ld, int64_t mat_stride, int m,
-    int p, int blk, int nblk, const T *__restrict__ linvt_re, const T *__restrict__ linvt_im,
-    double *__restrict__ work, double *__restrict__ out) {
+    int p, int B0, int bw, int nblk, const T *__restrict__ linvt_re,
+    const T *__restrict__ linvt_im, const double *__restrict__ work, double *__restrict__ out) {
+  const int v = blockIdx.x, f = v / p;
+  const T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
+  const double *yr = work + static_cast<int64_t>(2 * v) * m, *yi = yr + m;
+  double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
+  __shared__ double ysr[kSB], ysi[kSB], zsr[kB], zsi[kB], ptr_[4][kB], pti[4][kB];
+  const int tid = threadIdx.x;
+  ysr[tid] = tid < bw ? yr[B0 + tid] : 0.0;
+  ysi[tid] = tid < bw ? yi[B0 + tid] : 0.0;
+  __syncthreads();
+  const int i = tid & 63, q = tid >> 6;
+  for (int s0 = 0; s0 < bw; s0 += kB) {
+    const int sw = min(kB, bw - s0), blk = (B0 + s0) / kB;
+    const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
+    // z_i = sum_j Linv[i][j] y_j, Linv[i][j] = linvt[j][i] (unit lower,
+    // zero outside the block)
+    double ar = 0.0, ai = 0.0;
+#pragma unroll 4
+    for (int jj = 0; jj < 16; ++jj) {
+      const int j = 16 * q + jj;
+      const double lr = static_cast<double>(linvt_re[lbase + j * kB + i]);
+      const double li = static_cast<double>(linvt_im[lbase + j * kB + i]);
+      const double y_r = ysr[s0 + j], y_i = ysi[s0 + j];
+      ar += lr * y_r - li * y_i;
+      ai += lr * y_i + li * y_r;
+    }
+    ptr_[q][i] = ar;
+    pti[q][i] = ai;
+    __syncthreads();
+    if (tid < kB) {
+      const double zr = (ptr_[0][tid] + ptr_[1][tid]) + (ptr_[2][tid] + ptr_[3][tid]);
+      const double zi = (pti[0][tid] + pti[1][tid]) + (pti[2][tid] + pti[3][tid]);
+      zsr[tid] = zr;
+      zsi[tid] = zi;
+      if (tid < sw) {
+        orr[B0 + s0 + tid] = zr;
+        oi[B0 + s0 + tid] = zi;
+      }
+    }
+    __syncthreads();
+    // later rows of the super-block: y[r] -= sum_c L[r][s0 + c] z[c]
+    const int r = s0 + sw + tid;
+    if (r < bw) {
+      double ur = ysr[r], ui = ysi[r];
+      for (int c = 0; c < sw; ++c) {
+        const int64_t a = static_cast<int64_t>(B0 + s0 + c) * ld + B0 + r;
+        const double lr = static_cast<double>(Re[a]), li = static_cast<double>(Im[a]);
+        ur -= lr * zsr[c] - li * zsi[c];
+        ui -= lr * zsi[c] + li * zsr[c];
+      }
+      ysr[r] = ur;
+      ysi[r] = ui;
+    }
+    __syncthreads();
+  }
+}
+
+// Forward, rows below the super-block: work[r] -= sum_c L[r][B0 + c] out[B0 + c].
+template <typename T>
+__global__ void __launch_bounds__(256) fwd_update_kernel(
+    const T *__restrict__ re, const T *__restrict__ im, int64_t ld, int64_t mat_stride, int m,
+    int p, int B0, int bw, double *__restrict__ work, const double *__restrict__ out) {
   const int v = blockIdx.y, f = v / p;
   const T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
   double *yr = work + static_cast<int64_t>(2 * v) * m, *yi = yr + m;
-  double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
-  const int b0 = blk * kB, bw = min(kB, m - b0);
-  __shared__ double inr[kB], ini[kB], zr[kB], zi[kB];
+  const double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
+  __shared__ double zr[kSB], zi[kSB];
   const int tid = threadIdx.x;
-  if (tid < kB) {
-    inr[tid] = tid < bw ? yr[b0 + tid] : 0.0;
-    ini[tid] = tid < bw ? yi[b0 + tid] : 0.0;
-  }
+  zr[tid] = tid < bw ? orr[B0 + tid] : 0.0;
+  zi[tid] = tid < bw ? oi[B0 + tid] : 0.0;
   __syncthreads();
-  if (tid < kB) {
-    // z_i = sum_{j <= i} Linv[i][j] y_j, Linv[i][j] = linvt[j][i]
-    double ar = 0.0, ai = 0.0;
-    for (int j = 0; j <= tid && j < bw; ++j) {
-      const double lr = static_cast<double>(linvt_re[lbase + j * kB + tid]);
-      const double li = static_cast<double>(linvt_im[lbase + j * kB + tid]);
-      const double lrr = (j == tid) ? 1.0 : lr, lii = (j == tid) ? 0.0 : li;
-      ar += lrr * inr[j] - lii * ini[j];
-      ai += lrr * ini[j] + lii * inr[j];
-    }
-    zr[tid] = ar;
-    zi[tid] = ai;
-    if (blockIdx.x == 0 && tid < bw) {
-      orr[b0 + tid] = ar;
-      oi[b0 + tid] = ai;
-    }
+  const int r = B0 + bw + blockIdx.x * 256 + tid;
+  if (r >= m) return;
+  double ar = yr[r], ai = yi[r];
+  const T *cr = Re + static_cast<int64_t>(B0) * ld + r, *ci = Im + static_cast<int64_t>(B0) * ld + r;
+  // deep unroll: many independent column loads in flight per thread
+#pragma unroll 32
+  for (int c = 0; c < bw; ++c) {
+    const double lr = static_cast<double>(cr[c * ld]), li = static_cast<double>(ci[c * ld]);
+    ar -= lr * zr[c] - li * zi[c];
+    ai -= lr * zi[c] + li * zr[c];
   }
-  __syncthreads();
-  const int64_t r = static_cast<int64_t>(b0) + bw + blockIdx.x * static_cast<int64_t>(blockDim.x) + tid;
-  if (r < m) {
-    double ar = yr[r], ai = yi[r];
-    for (int c = 0; c < bw; ++c) {
-      const int64_t a = (b0 + c) * ld + r;
-      const double lr = static_cast<double>(Re[a]), li = static_cast<double>(Im[a]);
-      ar -= lr * zr[c] - li * zi[c];
-      ai -= lr * zi[c] + li * zr[c];
-    }
-    yr[r] = ar;
-    yi[r] = ai;
-  }
+  yr[r] = ar;
+  yi[r] = ai;
 }
 
 // z = y D^-1 (D on the factor's diagonal), from `in` into `out`.
@@ -739,74 +818,143 @@ __global__ void dscale_kernel(const T *__restrict__ re, const T *__restrict__ im
   out[o + m] = y.im;
 }
 
-// Backward, block blk (last to first): x_b := Linv_b^T z_b written to out;
-// columns i < b0: z[i] -= sum_c L[b0 + c][i] x_b[c].
+// Backward, diagonal part of the super-block at B0: solves L_BB^T x = z_B
+// for z_B = work[B0, B0 + bw) (sub-blocks last to first) and writes x to out.
 template <typename T>
-__global__ void __launch_bounds__(256) bwd_block_kernel(
+__global__ void __launch_bounds__(256) bwd_diag_kernel(
     const T *__restrict__ re, const T *__restrict__ im, int64_t ld, int64_t mat_stride, int m,
-    int p, int blk, int nblk, const T *__restrict__ linvt_re, const T *__restrict__ linvt_im,
-    double *__restrict__ work, double *__restrict__ out) {
+    int p, int B0, int bw, int nblk, const T *__restrict__ linvt_re,
+    const T *__restrict__ linvt_im, const double *__restrict__ work, double *__restrict__ out) {
+  const int v = blockIdx.x, f = v / p;
+  const T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
+  const double *zr_g = work + static_cast<int64_t>(2 * v) * m, *zi_g = zr_g + m;
+  double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
+  __shared__ double zsr[kSB], zsi[kSB], xsr[kB], xsi[kB], ptr_[4][kB], pti[4][kB];
+  const int tid = threadIdx.x;
+  zsr[tid] = tid < bw ? zr_g[B0 + tid] : 0.0;
+  zsi[tid] = tid < bw ? zi_g[B0 + tid] : 0.0;
+  __syncthreads();
+  const int c = tid & 63, q = tid >> 6;
+  for (int s0 = ((bw - 1) / kB) * kB; s0 >= 0; s0 -= kB) {
+    const int sw = min(kB, bw - s0), blk = (B0 + s0) / kB;
+    const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
+    // x_c = sum_i Linv[i][c] z_i, Linv[i][c] = linvt[c][i]
+    double ar = 0.0, ai = 0.0;
+#pragma unroll 4
+    for (int ii = 0; ii < 16; ++ii) {
+      const int i = 16 * q + ii;
+      const double lr = static_cast<double>(linvt_re[lbase + c * kB + i]);
+      const double li = static_cast<double>(linvt_im[lbase + c * kB + i]);
+      const double z_r = zsr[s0 + i], z_i = zsi[s0 + i];
+      ar += lr * z_r - li * z_i;
+      ai += lr * z_i + li * z_r;
+    }
+    ptr_[q][c] = ar;
+    pti[q][c] = ai;
+    __syncthreads();
+    if (tid < kB) {
+      const double xr = (ptr_[0][tid] + ptr_[1][tid]) + (ptr_[2][tid] + ptr_[3][tid]);
+      const double xi = (pti[0][tid] + pti[1][tid]) + (pti[2][tid] + pti[3][tid]);
+      xsr[tid] = xr;
+      xsi[tid] = xi;
+      if (tid < sw) {
+        orr[B0 + s0 + tid] = xr;
+        oi[B0 + s0 + tid] = xi;
+      }
+    }
+    __syncthreads();
+    // earlier rows of the super-block: z[i] -= sum_c L[s0 + c][i] x[c]
+    if (tid < s0) {
+      const int i = tid;
+      double ur = zsr[i], ui = zsi[i];
+      const int64_t base = static_cast<int64_t>(B0 + i) * ld + B0 + s0;
+      for (int cc = 0; cc < sw; ++cc) {
+        const double lr = static_cast<double>(Re[base + cc]), li = static_cast<double>(Im[base + cc]);
+        ur -= lr * xsr[cc] - li * xsi[cc];
+        ui -= lr * xsi[cc] + li * xsr[cc];
+      }
+      zsr[i] = ur;
+      zsi[i] = ui;
+    }
+    __syncthreads();
+  }
+}
+
+// Backward, rows above the super-block: work[i] -= sum_c L[B0 + c][i] out[B0 + c]
+// for i < B0.  Workgroup of 64 rows i; L[B0 + c][i] = A(row B0 + c, col i)
+// is staged as 64 x 64 tiles, read along c (contiguous), used along i.
+template <typename T>
+__global__ void __launch_bounds__(256) bwd_update_kernel(
+    const T *__restrict__ re, const T *__restrict__ im, int64_t ld, int64_t mat_stride, int m,
+    int p, int B0, int bw, double *__restrict__ work, const double *__restrict__ out) {
   const int v = blockIdx.y, f = v / p;
   const T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
   double *zr_g = work + static_cast<int64_t>(2 * v) * m, *zi_g = zr_g + m;
-  double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
-  const int b0 = blk * kB, bw = min(kB, m - b0);
-  __shared__ double inr[kB], ini[kB], xr[kB], xi[kB];
-  const int tid = threadIdx.x;
-  if (tid < kB) {
-    inr[tid] = tid < bw ? zr_g[b0 + tid] : 0.0;
-    ini[tid] = tid < bw ? zi_g[b0 + tid] : 0.0;
+  const double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
+  __shared__ double xr[kSB], xi[kSB], ptr_[4][kB], pti[4][kB];
+  __shared__ T tr_[kB][kB + 1], ti_[kB][kB + 1];  // [i][c]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  xr[tid] = tid < bw ? orr[B0 + tid] : 0.0;
+  xi[tid] = tid < bw ? oi[B0 + tid] : 0.0;
+  const int i0 = blockIdx.x * kB;
+  double ar = 0.0, ai = 0.0;
+  for (int c0 = 0; c0 < bw; c0 += kB) {
+    __syncthreads();
+    const int c = min(c0 + lane, bw - 1);
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+      const int ii = 16 * wave + k;
+      const int i = min(i0 + ii, B0 - 1);
+      const int64_t a = static_cast<int64_t>(i) * ld + B0 + c;
+      tr_[ii][lane] = Re[a];
+      ti_[ii][lane] = Im[a];
+    }
+    __syncthreads();
+    // thread (i = lane, quarter = wave): 16 columns of this tile
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+      const int cc = 16 * wave + k;
+      const double lr = static_cast<double>(tr_[lane][cc]), li = static_cast<double>(ti_[lane][cc]);
+      const double x_r = xr[c0 + cc], x_i = xi[c0 + cc];  // zero past bw
+      ar += lr * x_r - li * x_i;
+      ai += lr * x_i + li * x_r;
+    }
   }
+  ptr_[wave][lane] = ar;
+  pti[wave][lane] = ai;
   __syncthreads();
-  if (tid < kB) {
-    // x_c = sum_{i >= c} Linv[i][c] z_i, Linv[i][c] = linvt[c][i]
-    double ar = 0.0, ai = 0.0;
-    for (int i = tid; i < bw; ++i) {
-      const double lr = (i == tid) ? 1.0 : static_cast<double>(linvt_re[lbase + tid * kB + i]);
-      const double li = (i == tid) ? 0.0 : static_cast<double>(linvt_im[lbase + tid * kB + i]);
-      ar += lr * inr[i] - li * ini[i];
-      ai += lr * ini[i] + li * inr[i];
-    }
-    xr[tid] = ar;
-    xi[tid] = ai;
-    if (blockIdx.x == 0 && tid < bw) {
-      orr[b0 + tid] = ar;
-      oi[b0 + tid] = ai;
-    }
-  }
-  __syncthreads();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + tid;
-  if (i < b0) {
-    double ar = zr_g[i], ai = zi_g[i];
-    const int64_t base = i * ld + b0;  // L[b0 + c][i] = A[i * ld + b0 + c]
-    for (int c = 0; c < bw; ++c) {
-      const double lr = static_cast<double>(Re[base + c]), li = static_cast<double>(Im[base + c]);
-      ar -= lr * xr[c] - li * xi[c];
-      ai -= lr * xi[c] + li * xr[c];
-    }
-    zr_g[i] = ar;
-    zi_g[i] = ai;
+  const int i = i0 + tid;
+  if (tid < kB && i < B0) {
+    zr_g[i] -= (ptr_[0][tid] + ptr_[1][tid]) + (ptr_[2][tid] + ptr_[3][tid]);
+    zi_g[i] -= (pti[0][tid] + pti[1][tid]) + (pti[2][tid] + pti[3][tid]);
   }
 }
 
 // P[q][i] = sum_j L[j][i] X[q][j], i < m, over Q = 2 K p vectors; L is the
 // symmetric FP64 l_mesh (leading dimension lds), read column-contiguously.
+// The j range is cut into `slices` (so the launch fills the chip); slice s
+// writes its partial sums to part[s][q][i] and lx_reduce_kernel adds the
+// slices in order.
 __global__ void __launch_bounds__(256) lx_kernel(const double *__restrict__ L, int64_t lds, int m,
                                                  int Q, const double *__restrict__ X,
-                                                 double *__restrict__ P) {
-  const int i0 = blockIdx.x * 64, q0 = blockIdx.y * 64;
+                                                 double *__restrict__ part, int slice_len) {
+  const int i0 = blockIdx.x * 64, q0 = blockIdx.y * 64, sl = blockIdx.z;
+  const int jb = sl * slice_len, je = min(m, jb + slice_len);
   __shared__ double ls[32][65], xs[32][65];
   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   double acc[4][4] = {};
-  for (int j0 = 0; j0 < m; j0 += 32) {
+  for (int j0 = jb; j0 < je; j0 += 32) {
     __syncthreads();
     for (int q = threadIdx.x; q < 32 * 64; q += blockDim.x) {
       const int j = q / 64, x = q % 64;
       const int jj = j0 + j;
-      ls[j][x] = (jj < m && i0 + x < m) ? L[static_cast<int64_t>(jj) * lds + i0 + x] : 0.0;
+      const bool lok = jj < je && i0 + x < m;
+      const double lv = L[static_cast<int64_t>(min(jj, m - 1)) * lds + min(i0 + x, m - 1)];
+      ls[j][x] = lok ? lv : 0.0;
       const int xq = q / 32, xj = q % 32;
-      xs[xj][xq] = (j0 + xj < m && q0 + xq < Q) ? X[static_cast<int64_t>(q0 + xq) * m + j0 + xj] : 0.0;
+      const bool xok = j0 + xj < je && q0 + xq < Q;
+      const double xv = X[static_cast<int64_t>(min(q0 + xq, Q - 1)) * m + min(j0 + xj, m - 1)];
+      xs[xj][xq] = xok ? xv : 0.0;
     }
     __syncthreads();
     for (int j = 0; j < 32; ++j) {
@@ -817,11 +965,21 @@ __global__ void __launch_bounds__(256) lx_kernel(const double *__restrict__ L, i
         for (int s = 0; s < 4; ++s) acc[u][s] = __builtin_fma(a[u], b[s], acc[u][s]);
     }
   }
+  double *out = part + static_cast<int64_t>(sl) * Q * m;
   for (int u = 0; u < 4; ++u)
     for (int s = 0; s < 4; ++s) {
       const int i = i0 + tx + 16 * u, q = q0 + ty + 16 * s;
-      if (i < m && q < Q) P[static_cast<int64_t>(q) * m + i] = acc[u][s];
+      if (i < m && q < Q) out[static_cast<int64_t>(q) * m + i] = acc[u][s];
     }
+}
+
+__global__ void lx_reduce_kernel(int64_t n, int slices, const double *__restrict__ part,
+                                 double *__restrict__ P) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  double a = part[t];
+  for (int s = 1; s < slices; ++s) a += part[s * n + t];
+  P[t] = a;
 }
 
 // r = b - (R + j w L) x with b = Zep: r_re = b_re - (R x)_re + w P_im,
@@ -898,6 +1056,36 @@ __global__ void __launch_bounds__(256) functional_kernel(
     const double corr = __builtin_sqrt(red[2][0] * red[2][0] + red[3][0] * red[3][0]);
     const double zz = __builtin_sqrt(zr * zr + zi * zi);
     metric[e] = zz > 0.0 ? corr / zz : corr;
+  }
+}
+
+// Squared norms of the last correction dx and of the residual r of every
+// vector (re and im rows adjacent): nrm[2 v] = |dx_v|^2, nrm[2 v + 1] =
+// |r_v|^2; fixed-order reduction.
+__global__ void __launch_bounds__(256) norms_kernel(int m, const double *__restrict__ DX,
+                                                    const double *__restrict__ Rv,
+                                                    double *__restrict__ nrm) {
+  const int v = blockIdx.x;
+  const double *d = DX + static_cast<int64_t>(2 * v) * m, *r = Rv + static_cast<int64_t>(2 * v) * m;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < 2 * m; i += blockDim.x) {
+    a += d[i] * d[i];
+    b += r[i] * r[i];
+  }
+  __shared__ double ra[256], rb[256];
+  ra[threadIdx.x] = a;
+  rb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      ra[threadIdx.x] += ra[threadIdx.x + s];
+      rb[threadIdx.x] += rb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    nrm[2 * v] = ra[0];
+    nrm[2 * v + 1] = rb[0];
   }
 }
 
@@ -989,26 +1177,33 @@ int64_t split_blocks(const Layout &L) {
   return split_block(KC, 2 * KC, G);
 }
 
-void launch_left_update(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K, int J,
-                        int nbj, hipStream_t stream, Profiler *prof) {
+fh_status launch_left_update(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K,
+                             int J, int nbj, int kc_beg, int kc_end, hipStream_t stream,
+                             Profiler *prof) {
   // the kernel takes k in pairs of 32-column chunks
-  if (J % 64 != 0) return;
+  if (kc_end <= kc_beg || (kc_end - kc_beg) % 2 != 0)
+    return set_error(FH_E_INVALID_ARGUMENT, "left-looking update: odd chunk range");
   const int rtiles = (L.N - J + 127) / 128, ctiles = (nbj + 127) / 128;
   const int T = ctiles * (ctiles - 1) / 2 + (rtiles - (ctiles - 1)) * ctiles;
   if (prof) prof->begin(stream);
   update_left_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
-      F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, sb.l, sb.w, sb.stride);
+      F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, kc_beg, kc_end, sb.l,
+      sb.w, sb.stride);
   if (prof) {
     double entries = 0.0;
     for (int s = J; s < J + nbj; ++s) entries += static_cast<double>(L.N - s);
-    prof->end(stream, 8.0 * J * entries * K, 3.0);
+    prof->end(stream, 8.0 * 32.0 * (kc_end - kc_beg) * entries * K, 3.0);
   }
+  FH_HIP_TRY(hipGetLastError());
+  return FH_OK;
 }
 
+// Split copies of the factor columns [k0, k0 + width) (width a multiple of
+// 32), rows >= k0 + width; W is read from the buffer of the outer block at J.
 void launch_split_panel(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K, int J,
-                        int nbj, hipStream_t stream) {
-  const int kc0 = J / 32, kcn = nbj / 32;
-  const int g0 = (J + nbj) / 16, gn = sb.G - g0;
+                        int k0, int width, hipStream_t stream) {
+  const int kc0 = k0 / 32, kcn = width / 32;
+  const int g0 = (k0 + width) / 16, gn = sb.G - g0;
   if (kcn <= 0 || gn <= 0) return;
   const int items = kcn * gn;
   split_panel_kernel<<<dim3((items + 3) / 4, K), 256, 0, stream>>>(
@@ -1025,9 +1220,9 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
                     const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
                     const SplitBufs *sb = nullptr) {
   {
-    dim3 grid(2048, K);
-    form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, L.N,
-                                             L.ld, F.re, F.im, L.mat_stride);
+    dim3 grid((L.N + 255) / 256, L.N);
+    form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, K,
+                                             L.N, L.ld, F.re, F.im, L.mat_stride);
     FH_HIP_TRY(hipGetLastError());
     if (in.r_nnz) {
       const size_t blocks = std::min<size_t>((in.r_nnz + 255) / 256, 4096);
@@ -1042,13 +1237,22 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
     const int KE = std::min(K0 + L.nb, L.m);
     if constexpr (std::is_same<T, float>::value) {
       if (sb && K0 > 0) {
-        launch_left_update(L, F, *sb, K, K0, KE - K0, stream, prof);
-        FH_HIP_TRY(hipGetLastError());
+        const fh_status st = launch_left_update(L, F, *sb, K, K0, KE - K0, 0, K0 / 32, stream, prof);
+        if (st) return st;
       }
     }
     for (int k0 = K0; k0 < KE; k0 += kB) {
       const int w = std::min(kB, KE - k0);
       const int blk = k0 / kB;
+      if constexpr (std::is_same<T, float>::value) {
+        // left-looking inside the outer block too: the earlier 64-column
+        // blocks of this outer block, from their split copies
+        if (sb && k0 > K0) {
+          const fh_status st =
+              launch_left_update(L, F, *sb, K, k0, w, K0 / 32, k0 / 32, stream, prof);
+          if (st) return st;
+        }
+      }
       diag_kernel<T><<<K, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
                                             F.lt_re, F.lt_im, F.di_re, F.di_im, d_singular);
       FH_HIP_TRY(hipGetLastError());
@@ -1059,17 +1263,20 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
           F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk, L.N, F.wre, F.wim, L.w_stride,
           k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im);
       FH_HIP_TRY(hipGetLastError());
+      if constexpr (std::is_same<T, float>::value) {
+        if (sb) {
+          if (k0 + w < L.m) launch_split_panel(L, F, *sb, K, K0, k0, w, stream);
+          FH_HIP_TRY(hipGetLastError());
+          continue;
+        }
+      }
       if (k0 + w < KE)
         launch_update<T>(L, F, K, k0 + w, k0 + w, KE, k0, k0 - K0, w, false, stream, nullptr);
       FH_HIP_TRY(hipGetLastError());
     }
     if (KE < L.N) {
       if constexpr (std::is_same<T, float>::value) {
-        if (sb) {
-          if (KE < L.m) launch_split_panel(L, F, *sb, K, K0, KE - K0, stream);
-          FH_HIP_TRY(hipGetLastError());
-          continue;
-        }
+        if (sb) continue;
       }
       launch_update<T>(L, F, K, KE, KE, L.N, K0, 0, KE - K0, true, stream, prof);
     }
@@ -1086,24 +1293,27 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
   const int m = L.m;
   const size_t n = static_cast<size_t>(2) * V * m;
   FH_HIP_TRY(hipMemcpyAsync(work, vec, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  for (int blk = 0; blk < L.nblk; ++blk) {
-    const int b0 = blk * kB, bw = std::min(kB, m - b0);
-    const int below = m - b0 - bw;
-    const int chunks = std::max(1, (below + 255) / 256);
-    fwd_block_kernel<T><<<dim3(chunks, V), 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m,
-                                                             L.p, blk, L.nblk, F.lt_re, F.lt_im,
-                                                             work, vec);
+  const int nsb = (m + kSB - 1) / kSB;
+  for (int sb = 0; sb < nsb; ++sb) {
+    const int B0 = sb * kSB, bw = std::min(kSB, m - B0);
+    fwd_diag_kernel<T><<<V, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw,
+                                              L.nblk, F.lt_re, F.lt_im, work, vec);
+    const int below = m - B0 - bw;
+    if (below > 0)
+      fwd_update_kernel<T><<<dim3((below + 255) / 256, V), 256, 0, stream>>>(
+          F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw, work, vec);
   }
   FH_HIP_TRY(hipGetLastError());
   dscale_kernel<T><<<static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256), 256, 0, stream>>>(
       F.re, F.im, L.ld, L.mat_stride, m, L.p, V, vec, work);
   FH_HIP_TRY(hipGetLastError());
-  for (int blk = L.nblk - 1; blk >= 0; --blk) {
-    const int b0 = blk * kB;
-    const int chunks = std::max(1, (b0 + 255) / 256);
-    bwd_block_kernel<T><<<dim3(chunks, V), 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m,
-                                                             L.p, blk, L.nblk, F.lt_re, F.lt_im,
-                                                             work, vec);
+  for (int sb = nsb - 1; sb >= 0; --sb) {
+    const int B0 = sb * kSB, bw = std::min(kSB, m - B0);
+    bwd_diag_kernel<T><<<V, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw,
+                                              L.nblk, F.lt_re, F.lt_im, work, vec);
+    if (B0 > 0)
+      bwd_update_kernel<T><<<dim3((B0 + kB - 1) / kB, V), 256, 0, stream>>>(
+          F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw, work, vec);
   }
   FH_HIP_TRY(hipGetLastError());
   return FH_OK;
@@ -1117,7 +1327,7 @@ size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined) {
   in.p = p;
   if (!refined) return factor_bytes<double>(layout_of(in, kNbPlain), 1);
   const Layout L = layout_of(in, kNbRefined);
-  return factor_bytes<float>(L, 1) + static_cast<size_t>(p) * m * 2 * 4 * sizeof(double) +
+  return factor_bytes<float>(L, 1) + static_cast<size_t>(p) * m * 2 * 20 * sizeof(double) +
          2 * static_cast<size_t>(split_blocks(L)) * 4096;
 }
 
@@ -1153,7 +1363,12 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   fh_status st;
   const size_t fbytes = factor_bytes<float>(L, K);
   const size_t vbytes = static_cast<size_t>(Q) * m * sizeof(double);
-  const size_t extra = 4 * vbytes + static_cast<size_t>(K) * p * p * sizeof(double) +
+  const int itiles = (m + 63) / 64, qtiles = (Q + 63) / 64;
+  const int lx_slices = std::max(1, std::min(16, 2048 / std::max(1, itiles * qtiles)));
+  const int lx_slice_len = ((m + lx_slices - 1) / lx_slices + 31) / 32 * 32;
+  const size_t extra = (4 + static_cast<size_t>(lx_slices)) * vbytes +
+                       static_cast<size_t>(K) * p * p * sizeof(double) +
+                       2 * static_cast<size_t>(V) * sizeof(double) +
                        static_cast<size_t>(K) * sizeof(int) + 256;
   const size_t sbytes = static_cast<size_t>(split_blocks(L)) * 4096 * K;
   if ((st = ws.reserve(fbytes + extra + 2 * sbytes + 512))) return st;
@@ -1172,13 +1387,23 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   double *DX = R + static_cast<size_t>(Q) * m;
   double *P = DX + static_cast<size_t>(Q) * m;
   double *metric = P + static_cast<size_t>(Q) * m;
-  int *d_sing = reinterpret_cast<int *>(metric + static_cast<size_t>(K) * p * p);
+  double *lx_part = metric + static_cast<size_t>(K) * p * p;
+  double *d_nrm = lx_part + static_cast<size_t>(lx_slices) * Q * m;
+  int *d_sing = reinterpret_cast<int *>(d_nrm + 2 * static_cast<size_t>(V));
   const size_t n = static_cast<size_t>(Q) * m;
   const unsigned vec_blocks = static_cast<unsigned>((n + 255) / 256);
   const unsigned res_blocks = static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256);
   const int64_t lds = L.N;
-  std::vector<double> h_metric(static_cast<size_t>(K) * p * p);
-  const double kTol = 1e-13;
+  std::vector<double> h_metric(static_cast<size_t>(K) * p * p), h_nrm(2 * static_cast<size_t>(V));
+  std::vector<double> h_z(2 * static_cast<size_t>(K) * p * p);
+  // Acceptance.  With e = x* - x the error of x and r = b - Zee x = Zee e,
+  // the corrected Z = Zpp - b^T x - x^T r misses Z* by exactly -r^T e, so
+  // |dZ|_F <= |E|_F |R|_F.  While the iteration contracts by
+  // rho = |dx_k| / |dx_(k-1)| <= 1/2, |e_k| <= rho / (1 - rho) |dx_k|.
+  // Converged when that bound is below kZTol relative to |Z|_F (100x under
+  // the 1e-10 target), or when the first-order term |x^T r| / |Z| is below
+  // kTol.
+  const double kTol = 1e-13, kZTol = 1e-12;
   // Split-bf16 outer updates first; the FP32 update if that does not converge.
   for (int attempt = 0; attempt < 2; ++attempt) {
     const bool split = attempt == 0;
@@ -1197,18 +1422,25 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     out.refine_steps.assign(K, 0);
     out.refine_change.assign(K, INFINITY);
     out.converged.assign(K, 0);
-    std::vector<double> prev(K, INFINITY);
+    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0);
     for (int it = 1; it <= kMaxRefine; ++it) {
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
       if ((st = factor_solve<float>(L, F, V, DX, P, stream))) return st;
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
-      lx_kernel<<<dim3((m + 63) / 64, (Q + 63) / 64), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X, P);
+      lx_kernel<<<dim3(itiles, qtiles, lx_slices), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X,
+                                                                     lx_part, lx_slice_len);
+      lx_reduce_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), lx_slices, lx_part, P);
       residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
                                                       in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
                                                         in.d_lmesh, lds, X, R, d_z, metric);
+      norms_kernel<<<V, 256, 0, stream>>>(m, DX, R, d_nrm);
       FH_HIP_TRY(hipGetLastError());
       FH_HIP_TRY(hipMemcpyAsync(h_metric.data(), metric, h_metric.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, stream));
+      FH_HIP_TRY(hipMemcpyAsync(h_nrm.data(), d_nrm, h_nrm.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, stream));
+      FH_HIP_TRY(hipMemcpyAsync(h_z.data(), d_z, h_z.size() * sizeof(double),
                                 hipMemcpyDeviceToHost, stream));
       FH_HIP_TRY(hipStreamSynchronize(stream));
       bool all = true;
@@ -1220,10 +1452,29 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
         if (!(worst == worst)) worst = INFINITY;
         out.refine_steps[f] = it;
         out.refine_change[f] = worst;
-        // Converged when the first-order correction is negligible, or when it
-        // has stopped shrinking at a level the corrected Z absorbs.
-        if (worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f]))) out.converged[f] = 1;
-        else all = false;
+        double sdx = 0.0, sr = 0.0, sz = 0.0;
+        for (int q = 0; q < p; ++q) {
+          sdx += h_nrm[2 * (static_cast<size_t>(f) * p + q)];
+          sr += h_nrm[2 * (static_cast<size_t>(f) * p + q) + 1];
+        }
+        for (int q = 0; q < 2 * p * p; ++q) {
+          const double zq = h_z[2 * static_cast<size_t>(f) * p * p + q];
+          sz += zq * zq;
+        }
+        const double ndx = std::sqrt(sdx), nr = std::sqrt(sr), nz = std::sqrt(sz);
+        double bound = INFINITY;
+        if (it >= 2 && prev_dx[f] > 0.0) {
+          const double rho = ndx / prev_dx[f];
+          if (rho <= 0.5 && nz > 0.0) bound = rho / (1.0 - rho) * ndx * nr / nz;
+        }
+        prev_dx[f] = ndx;
+        // Converged when the bound on the error of Z is small, when the
+        // first-order term is negligible, or when that term has stopped
+        // shrinking at a level the corrected Z absorbs.
+        if (bound <= kZTol || worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f])))
+          out.converged[f] = 1;
+        else
+          all = false;
         prev[f] = worst;
       }
       if (all) break;
