@@ -98,6 +98,27 @@ __device__ __forceinline__ Z2 zrecip(Z2 d) {
   return {d.re / n, -d.im / n};
 }
 
+// bf16 splitting for the matrix-core products of the FP32 factorization
+// (see the left-looking update below): x = hi + lo in bf16.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
+  s16x8 s = __builtin_bit_cast(s16x8, v);
+  s ^= static_cast<short>(0x8000);
+  return __builtin_bit_cast(bf16x8, s);
+}
+
+// Block index of the split operand copies (see the left-looking update).
+__host__ __device__ __forceinline__ int64_t split_block(int kc, int g, int G) {
+  return static_cast<int64_t>(kc) * G - static_cast<int64_t>(kc) * (kc - 1) + (g - 2 * kc);
+}
+
 // ---------------------------------------------------------------------------
 // Forming Z(w) = R + j w L on the lower triangle (solve.rs:1003-1009), all
 // frequencies of the batch per thread: l_mesh is read once.  l_mesh is
@@ -318,7 +339,10 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
                                                     const T *__restrict__ linvt_re,
                                                     const T *__restrict__ linvt_im,
                                                     const T *__restrict__ dinv_re,
-                                                    const T *__restrict__ dinv_im) {
+                                                    const T *__restrict__ dinv_im,
+                                                    bf16x8 *__restrict__ lsplit,
+                                                    bf16x8 *__restrict__ wsplit,
+                                                    int64_t split_stride, int G, int write_w) {
   const int f = blockIdx.y;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   T *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
@@ -357,13 +381,70 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
         const int64_t r = r0 + wj + 16 * jb + (lane & 15);
         if (c < w && r < N) {
           const T wr = acc.re[ib][jb][v], wim_ = acc.im[ib][jb][v];
-          Wr[(wcol + c) * ld + r] = wr;
-          Wi[(wcol + c) * ld + r] = wim_;
+          if (write_w) {
+            Wr[(wcol + c) * ld + r] = wr;
+            Wi[(wcol + c) * ld + r] = wim_;
+          }
           const T dr = dinv_re[dbase + c], di = dinv_im[dbase + c];
           Re[(k0 + c) * ld + r] = wr * dr - wim_ * di;
           Im[(k0 + c) * ld + r] = wr * di + wim_ * dr;
         }
       }
+  if constexpr (std::is_same<T, float>::value) {
+    // Split copies of W and L for the left-looking update (w = 64): each
+    // wave's 32 x 32 part (one 32-column chunk, two 16-row groups) goes
+    // through LDS into the operand order, values as stored above.
+    if (lsplit) {
+      __shared__ float tsr[4][32][33], tsi[4][32][33];
+      const int kc = (k0 + wi) / 32, kg = lane >> 4;
+      for (int part = 0; part < 2; ++part) {
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+          for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int cc = 16 * ib + Mfma<T>::row(lane, v), rr = 16 * jb + (lane & 15);
+              const float wr = acc.re[ib][jb][v], wim_ = acc.im[ib][jb][v];
+              float vr = wr, vi = wim_;
+              if (part == 1) {
+                const float dr = dinv_re[dbase + wi + cc], di = dinv_im[dbase + wi + cc];
+                vr = wr * dr - wim_ * di;
+                vi = wr * di + wim_ * dr;
+              }
+              tsr[wave][rr][cc] = vr;
+              tsi[wave][rr][cc] = vi;
+            }
+        __syncthreads();
+        bf16x8 *dst = (part == 0 ? wsplit : lsplit) + f * split_stride;
+#pragma unroll
+        for (int gg = 0; gg < 2; ++gg) {
+          const int g = (r0 + wj) / 16 + gg;
+          if (g < G) {
+            const int rr = 16 * gg + (lane & 15);
+            const bool rok = r0 + wj + rr < N;
+            bf16x8 hr, lr, hi, li;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              __bf16 h, l;
+              split_bf16(rok ? tsr[wave][rr][8 * kg + e] : 0.f, h, l);
+              hr[e] = h;
+              lr[e] = l;
+              split_bf16(rok ? tsi[wave][rr][8 * kg + e] : 0.f, h, l);
+              hi[e] = h;
+              li[e] = l;
+            }
+            const int64_t base = split_block(kc, g, G) * 256 + lane;
+            dst[base] = hr;
+            dst[base + 64] = lr;
+            dst[base + 128] = hi;
+            dst[base + 192] = li;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -456,19 +537,6 @@ __global__ void __launch_bounds__(256) update_kernel(T *__restrict__ re, T *__re
 // term and the split residuals bound the relative error of every product by
 // about 3 * 2^-17; refinement in FP64 removes it.
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short s16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = static_cast<__bf16>(x);
-  lo = static_cast<__bf16>(x - static_cast<float>(hi));
-}
-
-__device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
-  s16x8 s = __builtin_bit_cast(s16x8, v);
-  s ^= static_cast<short>(0x8000);
-  return __builtin_bit_cast(bf16x8, s);
-}
 
 // ---------------------------------------------------------------------------
 // 3''. Left-looking form of the split-bf16 update.  Before the
@@ -487,56 +555,6 @@ __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
 // row 16 g + l % 16, columns 32 kc + 8 (l / 16) + e.  Block index
 // kc G - kc (kc - 1) + (g - 2 kc), G = ceil(N / 16).
 
-__host__ __device__ __forceinline__ int64_t split_block(int kc, int g, int G) {
-  return static_cast<int64_t>(kc) * G - static_cast<int64_t>(kc) * (kc - 1) + (g - 2 * kc);
-}
-
-// Split copies of L (lsplit) and W = L D (wsplit) for the chunks
-// [kc0, kc0 + kcn) of the outer block at J and the row groups [g0, G);
-// rows >= N are zero.  One wave per (chunk, group) block.
-__global__ void __launch_bounds__(256) split_panel_kernel(
-    const float *__restrict__ re, const float *__restrict__ im, int64_t ld, int64_t mat_stride,
-    const float *__restrict__ wre, const float *__restrict__ wim, int64_t w_stride, int J,
-    int kc0, int kcn, int g0, int gn, int N, int G, bf16x8 *__restrict__ lsplit,
-    bf16x8 *__restrict__ wsplit, int64_t split_stride) {
-  const int f = blockIdx.y;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int item = blockIdx.x * 4 + wave;  // (chunk, group) pair
-  if (item >= kcn * gn) return;
-  const int kc = kc0 + item / gn, g = g0 + item % gn;
-  const float *Re = re + f * mat_stride, *Im = im + f * mat_stride;
-  const float *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
-  const int row = 16 * g + (lane & 15);
-  const bool ok = row < N;
-  const int64_t rcl = min(row, N - 1);
-  const int kbase = 32 * kc + 8 * (lane >> 4);
-  bf16x8 lp[4], wp[4];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const int64_t k = kbase + e;
-    const float a = Re[k * ld + rcl], b = Im[k * ld + rcl];
-    const float c = Wr[(k - J) * ld + rcl], d = Wi[(k - J) * ld + rcl];
-    __bf16 h, l;
-    split_bf16(ok ? a : 0.f, h, l);
-    lp[0][e] = h;
-    lp[1][e] = l;
-    split_bf16(ok ? b : 0.f, h, l);
-    lp[2][e] = h;
-    lp[3][e] = l;
-    split_bf16(ok ? c : 0.f, h, l);
-    wp[0][e] = h;
-    wp[1][e] = l;
-    split_bf16(ok ? d : 0.f, h, l);
-    wp[2][e] = h;
-    wp[3][e] = l;
-  }
-  const int64_t base = f * split_stride + split_block(kc, g, G) * 4 * 64 + lane;
-#pragma unroll
-  for (int pl = 0; pl < 4; ++pl) {
-    lsplit[base + pl * 64] = lp[pl];
-    wsplit[base + pl * 64] = wp[pl];
-  }
-}
 
 // Tiles of the left-looking update: 128 x 128, rows r in [J, N), columns s
 // in [J, J + nbj); tile (tr, tc) with tc < ctiles and tr >= tc.  Listed row
@@ -773,6 +791,8 @@ __global__ void __launch_bounds__(256) fwd_diag_kernel(
 }
 
 // Forward, rows below the super-block: work[r] -= sum_c L[r][B0 + c] out[B0 + c].
+// 64 rows per workgroup, four threads per row (a quarter of the columns
+// each, loads issued in batches of 16), partial sums added in fixed order.
 template <typename T>
 __global__ void __launch_bounds__(256) fwd_update_kernel(
     const T *__restrict__ re, const T *__restrict__ im, int64_t ld, int64_t mat_stride, int m,
@@ -781,24 +801,38 @@ __global__ void __launch_bounds__(256) fwd_update_kernel(
   const T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   double *yr = work + static_cast<int64_t>(2 * v) * m, *yi = yr + m;
   const double *orr = out + static_cast<int64_t>(2 * v) * m, *oi = orr + m;
-  __shared__ double zr[kSB], zi[kSB];
-  const int tid = threadIdx.x;
-  zr[tid] = tid < bw ? orr[B0 + tid] : 0.0;
+  __shared__ double zr[kSB], zi[kSB], ptr_[4][kB], pti[4][kB];
+  const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+  zr[tid] = tid < bw ? orr[B0 + tid] : 0.0;  // zero past bw
   zi[tid] = tid < bw ? oi[B0 + tid] : 0.0;
   __syncthreads();
-  const int r = B0 + bw + blockIdx.x * 256 + tid;
-  if (r >= m) return;
-  double ar = yr[r], ai = yi[r];
-  const T *cr = Re + static_cast<int64_t>(B0) * ld + r, *ci = Im + static_cast<int64_t>(B0) * ld + r;
-  // deep unroll: many independent column loads in flight per thread
-#pragma unroll 32
-  for (int c = 0; c < bw; ++c) {
-    const double lr = static_cast<double>(cr[c * ld]), li = static_cast<double>(ci[c * ld]);
-    ar -= lr * zr[c] - li * zi[c];
-    ai -= lr * zi[c] + li * zr[c];
+  const int r = B0 + bw + blockIdx.x * kB + lane;
+  const int64_t rc = min(r, m - 1);
+  double ar = 0.0, ai = 0.0;
+#pragma unroll
+  for (int c0 = 0; c0 < kB; c0 += 16) {
+    T lr[16], li[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int64_t c = min(kB * q + c0 + e, bw - 1);
+      lr[e] = Re[(B0 + c) * ld + rc];
+      li[e] = Im[(B0 + c) * ld + rc];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int c = kB * q + c0 + e;
+      const double a = static_cast<double>(lr[e]), b = static_cast<double>(li[e]);
+      ar += a * zr[c] - b * zi[c];
+      ai += a * zi[c] + b * zr[c];
+    }
   }
-  yr[r] = ar;
-  yi[r] = ai;
+  ptr_[q][lane] = ar;
+  pti[q][lane] = ai;
+  __syncthreads();
+  if (q == 0 && r < m) {
+    yr[r] -= (ptr_[0][lane] + ptr_[1][lane]) + (ptr_[2][lane] + ptr_[3][lane]);
+    yi[r] -= (pti[0][lane] + pti[1][lane]) + (pti[2][lane] + pti[3][lane]);
+  }
 }
 
 // z = y D^-1 (D on the factor's diagonal), from `in` into `out`.
@@ -1198,19 +1232,6 @@ fh_status launch_left_update(const Layout &L, const Factor<float> &F, const Spli
   return FH_OK;
 }
 
-// Split copies of the factor columns [k0, k0 + width) (width a multiple of
-// 32), rows >= k0 + width; W is read from the buffer of the outer block at J.
-void launch_split_panel(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K, int J,
-                        int k0, int width, hipStream_t stream) {
-  const int kc0 = k0 / 32, kcn = width / 32;
-  const int g0 = (k0 + width) / 16, gn = sb.G - g0;
-  if (kcn <= 0 || gn <= 0) return;
-  const int items = kcn * gn;
-  split_panel_kernel<<<dim3((items + 3) / 4, K), 256, 0, stream>>>(
-      F.re, F.im, L.ld, L.mat_stride, F.wre, F.wim, L.w_stride, J, kc0, kcn, g0, gn, L.N, sb.G,
-      sb.l, sb.w, sb.stride);
-}
-
 // Factorizes (L D L^T) the first m pivots of the bordered matrices.  With
 // `sb` (FP32 only) the outer updates are left-looking on the split-bf16
 // copies; the trailing p x p block is then not updated (the refined form
@@ -1259,17 +1280,15 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
       const int rows = L.N - (k0 + w);
       if (rows <= 0) continue;
       const int ptiles = (rows + kT - 1) / kT;
+      // on the left-looking path the panel also writes the split copies of
+      // its W and L (needed when columns remain), and no FP32 W
+      const bool emit = sb && k0 + w < L.m;
       panel_kernel<T><<<dim3(ptiles, K), 256, 0, stream>>>(
           F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk, L.N, F.wre, F.wim, L.w_stride,
-          k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im);
+          k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im, emit ? sb->l : nullptr,
+          emit ? sb->w : nullptr, sb ? sb->stride : 0, sb ? sb->G : 0, sb ? 0 : 1);
       FH_HIP_TRY(hipGetLastError());
-      if constexpr (std::is_same<T, float>::value) {
-        if (sb) {
-          if (k0 + w < L.m) launch_split_panel(L, F, *sb, K, K0, k0, w, stream);
-          FH_HIP_TRY(hipGetLastError());
-          continue;
-        }
-      }
+      if (sb) continue;
       if (k0 + w < KE)
         launch_update<T>(L, F, K, k0 + w, k0 + w, KE, k0, k0 - K0, w, false, stream, nullptr);
       FH_HIP_TRY(hipGetLastError());
@@ -1300,7 +1319,7 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
                                               L.nblk, F.lt_re, F.lt_im, work, vec);
     const int below = m - B0 - bw;
     if (below > 0)
-      fwd_update_kernel<T><<<dim3((below + 255) / 256, V), 256, 0, stream>>>(
+      fwd_update_kernel<T><<<dim3((below + kB - 1) / kB, V), 256, 0, stream>>>(
           F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw, work, vec);
   }
   FH_HIP_TRY(hipGetLastError());
