@@ -101,16 +101,18 @@ __device__ __forceinline__ Z2 zrecip(Z2 d) {
 // bf16 splitting for the matrix-core products of the FP32 factorization
 // (see the left-looking update below): x = hi + lo in bf16.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short s16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   hi = static_cast<__bf16>(x);
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// -v: the sign bits of the eight bf16, one 32-bit XOR per pair
 __device__ __forceinline__ bf16x8 neg8(bf16x8 v) {
-  s16x8 s = __builtin_bit_cast(s16x8, v);
-  s ^= static_cast<short>(0x8000);
+  i32x4 s = __builtin_bit_cast(i32x4, v);
+  s ^= static_cast<int>(0x80008000u);
   return __builtin_bit_cast(bf16x8, s);
 }
 
@@ -594,8 +596,16 @@ __global__ void __launch_bounds__(256) update_left_kernel(
   const int lane = threadIdx.x & 63;
   const int wi = 64 * (wave >> 1), wj = 64 * (wave & 1);
   const int s0 = J + 128 * tc + wi, r0 = J + 128 * tr + wj;
-  const bf16x8 *Ls = lsplit + f * split_stride;
-  const bf16x8 *Ws = wsplit + f * split_stride;
+  // Operands through buffer loads: one descriptor per operand and
+  // frequency (scalar), a fixed per-lane offset, the block offset in a
+  // scalar register -- no per-lane 64-bit address arithmetic.
+  const __amdgpu_buffer_rsrc_t rL = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<bf16x8 *>(lsplit + f * split_stride), 0, static_cast<int>(split_stride * 16),
+      0x00020000);
+  const __amdgpu_buffer_rsrc_t rW = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<bf16x8 *>(wsplit + f * split_stride), 0, static_cast<int>(split_stride * 16),
+      0x00020000);
+  const int voff = lane * 16;
   int gs[4], gr[4];
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
@@ -612,16 +622,20 @@ __global__ void __launch_bounds__(256) update_left_kernel(
     }
   bf16x8 p[4][4], qa[4][4], qb[4][4];
   auto load_p = [&](int kc, int b) {
-    const bf16x8 *bp = Ls + split_block(kc, gs[b], G) * 256;
+    const int so = static_cast<int>(split_block(kc, gs[b], G) * 4096);
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) p[b][pl] = bp[64 * pl + lane];
+    for (int pl = 0; pl < 4; ++pl)
+      p[b][pl] = __builtin_bit_cast(
+          bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rL, voff + 1024 * pl, so, 0));
   };
   auto load_q = [&](int kc, bf16x8 (&q)[4][4]) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const bf16x8 *bq = Ws + split_block(kc, gr[b], G) * 256;
+      const int so = static_cast<int>(split_block(kc, gr[b], G) * 4096);
 #pragma unroll
-      for (int pl = 0; pl < 4; ++pl) q[b][pl] = bq[64 * pl + lane];
+      for (int pl = 0; pl < 4; ++pl)
+        q[b][pl] = __builtin_bit_cast(
+            bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rW, voff + 1024 * pl, so, 0));
     }
   };
   const int nkc = kc_end;
@@ -1423,8 +1437,11 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   // the 1e-10 target), or when the first-order term |x^T r| / |Z| is below
   // kTol.
   const double kTol = 1e-13, kZTol = 1e-12;
-  // Split-bf16 outer updates first; the FP32 update if that does not converge.
-  for (int attempt = 0; attempt < 2; ++attempt) {
+  // Split-bf16 outer updates first; the FP32 update if that does not converge
+  // (or if the split copies of one frequency exceed the 2 GiB range of a
+  // buffer descriptor, far above the dense path's 10 000 filaments).
+  const bool split_ok = split_blocks(L) * 4096 < (int64_t{1} << 31);
+  for (int attempt = split_ok ? 0 : 1; attempt < 2; ++attempt) {
     const bool split = attempt == 0;
     out.factor_arith = split ? 2 : 1;
     FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
