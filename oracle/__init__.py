@@ -318,3 +318,190 @@ class System:
 
 def solve(packed, freqs, threads: int = 0) -> np.ndarray:
     return System(packed, threads).sweep(freqs)
+
+
+# ---------------------------------------------------------------------------
+# pfft/*.rs, gmres.rs, iterative.rs
+
+FHO_E_NOT_CONVERGED = 9
+
+
+class PfftParams(C.Structure):
+    """PfftParams (pfft/mod.rs:183-231); defaults p = 5, k = 8, 8 cells per
+    filament."""
+    _fields_ = [("auto_grid", C.c_int), ("cells_per_filament", C.c_double),
+                ("spacing", C.c_double), ("near_field_radius", C.c_double),
+                ("interpolation_order", C.c_int)]
+
+    @classmethod
+    def default(cls, spacing: float = None, near_field_radius: float = 8.0, order: int = 5,
+                cells_per_filament: float = 8.0) -> "PfftParams":
+        p = cls()
+        p.auto_grid = 0 if spacing is not None else 1
+        p.cells_per_filament = cells_per_filament
+        p.spacing = spacing or 0.0
+        p.near_field_radius = near_field_radius
+        p.interpolation_order = order
+        return p
+
+
+_pfft_bound = False
+
+
+def _bind_pfft() -> C.CDLL:
+    global _pfft_bound
+    L = lib()
+    if not _pfft_bound:
+        vp, sz, dp = C.c_void_p, C.c_size_t, C.POINTER(C.c_double)
+        L.fho_pfft_create.restype = C.c_int
+        L.fho_pfft_create.argtypes = [vp, sz, C.POINTER(PfftParams), C.POINTER(vp)]
+        L.fho_pfft_destroy.argtypes = [vp]
+        L.fho_pfft_apply.argtypes = [vp, vp, vp]
+        L.fho_pfft_stats.argtypes = [vp, dp, vp, vp, C.POINTER(sz), C.POINTER(sz)]
+        L.fho_pfft_data.restype = C.c_int
+        L.fho_pfft_data.argtypes = [vp] * 8
+        L.fho_pfft_convolve.restype = C.c_int
+        L.fho_pfft_convolve.argtypes = [vp] * 5
+        L.fho_auto_spacing.restype = C.c_double
+        L.fho_auto_spacing.argtypes = [vp, C.c_double]
+        L.fho_smooth_size.restype = sz
+        L.fho_smooth_size.argtypes = [sz]
+        L.fho_axis_stencil.restype = sz
+        L.fho_axis_stencil.argtypes = [vp, C.c_double, vp, C.c_int, C.c_int, C.c_double, vp]
+        L.fho_gmres_dense.restype = C.c_int
+        L.fho_gmres_dense.argtypes = [sz, vp, vp, vp, C.c_double, sz, sz, C.c_int,
+                                      C.POINTER(sz), dp, C.POINTER(C.c_int)]
+        L.fho_iter_create.restype = C.c_int
+        L.fho_iter_create.argtypes = [vp, sz, vp, sz, vp, vp, sz, C.POINTER(PfftParams),
+                                      C.c_double, sz, sz, C.POINTER(vp)]
+        L.fho_iter_destroy.argtypes = [vp]
+        L.fho_iter_impedance.restype = C.c_int
+        L.fho_iter_impedance.argtypes = [vp, C.c_double, vp, vp]
+        L.fho_iter_counts.argtypes = [vp, C.POINTER(sz)]
+        L.fho_iter_operator.restype = vp
+        L.fho_iter_operator.argtypes = [vp]
+        _pfft_bound = True
+    return L
+
+
+def auto_spacing(extent, cells: float) -> float:
+    e = np.ascontiguousarray(extent, dtype=np.float64)
+    return _bind_pfft().fho_auto_spacing(_ptr(e), cells)
+
+
+def smooth_size(n: int) -> int:
+    return _bind_pfft().fho_smooth_size(n)
+
+
+def axis_stencil(origin, h: float, dims, m: int, axis: int, x: float):
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    d = np.ascontiguousarray(dims, dtype=np.uintp)
+    w = np.zeros(m)
+    first = _bind_pfft().fho_axis_stencil(_ptr(o), h, _ptr(d), m, axis, x, _ptr(w))
+    return first, w
+
+
+class Pfft:
+    """PfftOperator (pfft/mod.rs:270-528) of a FILAMENT_DTYPE array."""
+
+    def __init__(self, filaments: np.ndarray, params: PfftParams = None):
+        L = _bind_pfft()
+        self._f = np.ascontiguousarray(filaments)
+        self.n = len(self._f)
+        self.params = params or PfftParams.default()
+        self._h = C.c_void_p()
+        _check(L.fho_pfft_create(_ptr(self._f), self.n, C.byref(self.params), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _bind_pfft().fho_pfft_destroy(self._h)
+            self._h = None
+
+    def apply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        _bind_pfft().fho_pfft_apply(self._h, _ptr(x), _ptr(y))
+        return y
+
+    def stats(self) -> dict:
+        h = C.c_double()
+        dims = np.zeros(3, dtype=np.uintp)
+        padded = np.zeros(3, dtype=np.uintp)
+        pe, ne = C.c_size_t(), C.c_size_t()
+        _bind_pfft().fho_pfft_stats(self._h, C.byref(h), _ptr(dims), _ptr(padded), C.byref(pe),
+                                    C.byref(ne))
+        return dict(filaments=self.n, grid_spacing=h.value, grid_dims=[int(d) for d in dims],
+                    fft_dims=[int(d) for d in padded], projection_entries=pe.value,
+                    near_entries=ne.value)
+
+    def data(self) -> dict:
+        st = self.stats()
+        pe, ne = st["projection_entries"], st["near_entries"]
+        proj_off = np.zeros(self.n + 1, dtype=np.uintp)
+        proj_idx = np.zeros(max(pe, 1), dtype=np.uint32)
+        proj_w = np.zeros(max(pe, 1))
+        near_off = np.zeros(self.n + 1, dtype=np.uintp)
+        near_col = np.zeros(max(ne, 1), dtype=np.uint32)
+        near_val = np.zeros(max(ne, 1))
+        origin = np.zeros(3)
+        _bind_pfft().fho_pfft_data(self._h, _ptr(proj_off), _ptr(proj_idx), _ptr(proj_w),
+                                   _ptr(near_off), _ptr(near_col), _ptr(near_val), _ptr(origin))
+        return dict(proj_off=proj_off.astype(np.int64), proj_idx=proj_idx[:pe],
+                    proj_w=proj_w[:pe], near_off=near_off.astype(np.int64),
+                    near_col=near_col[:ne], near_val=near_val[:ne], origin=origin, **st)
+
+    def convolve(self, a, b=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        pa = np.zeros_like(a)
+        if b is not None:
+            b = np.ascontiguousarray(b, dtype=np.float64)
+            pb = np.zeros_like(b)
+        _check(_bind_pfft().fho_pfft_convolve(self._h, _ptr(a), _ptr(b) if b is not None else None,
+                                              _ptr(pa), _ptr(pb) if b is not None else None))
+        return (pa, pb) if b is not None else pa
+
+
+def gmres_dense(a: np.ndarray, b: np.ndarray, x0=None, tolerance: float = 1e-10,
+                restart: int = 100, max_iterations: int = 5000):
+    """gmres.rs on a dense matrix without preconditioning: (x, iterations,
+    relative_residual, converged)."""
+    complex_ = np.iscomplexobj(a) or np.iscomplexobj(b)
+    n = len(b)
+    A = np.asfortranarray(a, dtype=np.complex128)
+    B = np.ascontiguousarray(b, dtype=np.complex128)
+    X = np.zeros(n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+    it, rr, conv = C.c_size_t(), C.c_double(), C.c_int()
+    _check(_bind_pfft().fho_gmres_dense(n, _ptr(A), _ptr(B), _ptr(X), tolerance, restart,
+                                        max_iterations, 1 if complex_ else 0, C.byref(it),
+                                        C.byref(rr), C.byref(conv)))
+    return (X if complex_ else X.real.copy()), it.value, rr.value, bool(conv.value)
+
+
+class IterSystem:
+    """IterativeSystem (iterative.rs:155-536) of a PackedProblem."""
+
+    def __init__(self, packed, params: PfftParams = None, tolerance: float = 1e-10,
+                 restart: int = 100, max_iterations: int = 5000):
+        L = _bind_pfft()
+        self.params = params or PfftParams.default()
+        self._h = C.c_void_p()
+        _check(L.fho_iter_create(_ptr(packed.nodes), len(packed.nodes), _ptr(packed.segments),
+                                 len(packed.segments), _ptr(packed.grids), _ptr(packed.ports),
+                                 len(packed.ports), C.byref(self.params), tolerance, restart,
+                                 max_iterations, C.byref(self._h)))
+        c = (C.c_size_t * 6)()
+        L.fho_iter_counts(self._h, c)
+        (self.nodes, self.segments, self.filaments, self.meshes, self.internal,
+         self.ports) = list(c)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _bind_pfft().fho_iter_destroy(self._h)
+            self._h = None
+
+    def impedance(self, frequency: float):
+        p = self.ports
+        z = np.zeros((p, p), dtype=np.complex128)
+        its = np.zeros(max(p, 1), dtype=np.uintp)
+        _check(_bind_pfft().fho_iter_impedance(self._h, frequency, _ptr(z), _ptr(its)))
+        return z, [int(i) for i in its[:p]]

@@ -1897,3 +1897,1389 @@ int fho_solve(const double *nodes, size_t n_nodes, const fho_segment *segments,
   fho_system_destroy(s);
   return st;
 }
+
+/* ======================================================================== */
+/* pfft/{mod,grid,near,fft}.rs: the precorrected-FFT operator                */
+/* ======================================================================== */
+
+#define PFFT_QUAD_TOL 1e-10       /* grid.rs:41 */
+#define PFFT_QUAD_MAX 12          /* grid.rs:43 */
+#define PFFT_MAX_FFT_POINTS ((size_t)1 << 26) /* mod.rs:150 */
+
+struct fho_pfft {
+  size_t n;
+  int has_grid;
+  double origin[3], h;
+  size_t dims[3], padded[3];
+  int m;
+  double *dirs; /* n x 3 */
+  size_t *proj_off;
+  uint32_t *proj_idx;
+  double *proj_w;
+  size_t *spread_off;
+  uint32_t *spread_fil;
+  double *spread_w;
+  int comps[3], ncomp;
+  size_t *near_off;
+  uint32_t *near_col;
+  double *near_val;
+  double *spectrum; /* padded, natural order, real, includes 1/M */
+};
+
+/* mod.rs:614-631 */
+double fho_auto_spacing(const double extent[3], double cells) {
+  const double largest = fmax(fmax(extent[0], extent[1]), extent[2]);
+  if (cells <= 1.0 || largest <= 0.0) return fmax(largest, 2.2250738585072014e-308);
+  double small = largest / cells, large = largest;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = sqrt(small * large);
+    double count = 1.0;
+    for (int a = 0; a < 3; ++a) count *= fmax(extent[a] / mid, 1.0);
+    if (count > cells) small = mid;
+    else large = mid;
+  }
+  return large;
+}
+
+/* fft.rs:59-80 */
+size_t fho_smooth_size(size_t n) {
+  size_t m = n ? n : 1;
+  for (;;) {
+    size_t r = m;
+    while (r % 2 == 0) r /= 2;
+    while (r % 3 == 0) r /= 3;
+    while (r % 5 == 0) r /= 5;
+    if (r == 1) return m;
+    ++m;
+  }
+}
+
+/* grid.rs:95-116: first plane and the m Lagrange weights. */
+static size_t axis_stencil(const double origin[3], double h, const size_t dims[3], int m, int axis,
+                           double x, double *weights) {
+  const double u = (x - origin[axis]) / h;
+  double fl = floor(u - 0.5 * (double)m) + 1.0;
+  if (fl < 0.0) fl = 0.0;
+  size_t first = (size_t)fl;
+  if (first > dims[axis] - (size_t)m) first = dims[axis] - (size_t)m;
+  const double t = u - (double)first;
+  for (int k = 0; k < m; ++k) {
+    double value = 1.0;
+    for (int l = 0; l < m; ++l)
+      if (l != k) value *= (t - (double)l) / ((double)k - (double)l);
+    weights[k] = value;
+  }
+  return first;
+}
+
+size_t fho_axis_stencil(const double origin[3], double h, const size_t dims[3], int m, int axis,
+                        double x, double *weights) {
+  return axis_stencil(origin, h, dims, m, axis, x, weights);
+}
+
+typedef struct {
+  size_t count;
+  uint32_t *idx;
+  double *w;
+  size_t lo[3], hi[3];
+} projection;
+
+/* grid.rs:155-216.  Contributions are accumulated in node order (the
+ * reference merges them after an unstable sort of the indices, so its order
+ * of addition for a grid point is unspecified); entries are the grid points
+ * some node's stencil touches, in ascending index. */
+static int project_one(const struct fho_pfft *op, const fho_filament *f, double far,
+                       const double *thresholds, projection *out) {
+  const v3 axes[3] = {vload(f->dir), vload(f->wdir), vload(f->hdir)};
+  const double extents[3] = {f->length, f->width, f->height};
+  const v3 corner = vsub(vsub(vload(f->start), vmul(axes[1], 0.5 * f->width)),
+                         vmul(axes[2], 0.5 * f->height));
+  double *nx[3], *nw[3];
+  size_t cnt[3];
+  for (int d = 0; d < 3; ++d) {
+    double pc = ceil(extents[d] / op->h);
+    if (pc < 1.0) pc = 1.0;
+    const size_t pieces = (size_t)pc;
+    const double piece = extents[d] / (double)pieces;
+    int order = order_for(thresholds, PFFT_QUAD_MAX, 0.5 * piece, far);
+    if (!order) order = PFFT_QUAD_MAX;
+    cnt[d] = pieces * (size_t)order;
+    nx[d] = malloc(sizeof(double) * cnt[d]);
+    nw[d] = malloc(sizeof(double) * cnt[d]);
+    size_t q = 0;
+    for (size_t p = 0; p < pieces; ++p)
+      for (int i = 0; i < order; ++i) {
+        rule_on(order, i, (double)p * piece, (double)(p + 1) * piece, &nx[d][q], &nw[d][q]);
+        ++q;
+      }
+  }
+  const double inverse_area = 1.0 / area_of(f);
+  const int m = op->m;
+  double wx[16], wy[16], wz[16];
+  /* The union of the stencils: a box, found in a first pass. */
+  size_t lo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, hi[3] = {0, 0, 0};
+  for (size_t a = 0; a < cnt[0]; ++a)
+    for (size_t b = 0; b < cnt[1]; ++b)
+      for (size_t c = 0; c < cnt[2]; ++c) {
+        const v3 r = vadd(vadd(vadd(corner, vmul(axes[0], nx[0][a])), vmul(axes[1], nx[1][b])),
+                          vmul(axes[2], nx[2][c]));
+        const double rr[3] = {r.x, r.y, r.z};
+        for (int ax = 0; ax < 3; ++ax) {
+          const size_t first = axis_stencil(op->origin, op->h, op->dims, m, ax, rr[ax], wx);
+          if (first < lo[ax]) lo[ax] = first;
+          if (first + (size_t)m - 1 > hi[ax]) hi[ax] = first + (size_t)m - 1;
+        }
+      }
+  const size_t bd[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  const size_t blen = bd[0] * bd[1] * bd[2];
+  double *acc = calloc(blen, sizeof(double));
+  unsigned char *touched = calloc(blen, 1);
+  for (size_t a = 0; a < cnt[0]; ++a)
+    for (size_t b = 0; b < cnt[1]; ++b)
+      for (size_t c = 0; c < cnt[2]; ++c) {
+        const v3 r = vadd(vadd(vadd(corner, vmul(axes[0], nx[0][a])), vmul(axes[1], nx[1][b])),
+                          vmul(axes[2], nx[2][c]));
+        const double weight = nw[0][a] * nw[1][b] * nw[2][c] * inverse_area;
+        const size_t fx = axis_stencil(op->origin, op->h, op->dims, m, 0, r.x, wx);
+        const size_t fy = axis_stencil(op->origin, op->h, op->dims, m, 1, r.y, wy);
+        const size_t fz = axis_stencil(op->origin, op->h, op->dims, m, 2, r.z, wz);
+        for (int i = 0; i < m; ++i)
+          for (int j = 0; j < m; ++j) {
+            const double lxy = weight * wx[i] * wy[j];
+            for (int k = 0; k < m; ++k) {
+              const size_t cell =
+                  ((fx + i - lo[0]) * bd[1] + (fy + j - lo[1])) * bd[2] + (fz + k - lo[2]);
+              acc[cell] += lxy * wz[k];
+              touched[cell] = 1;
+            }
+          }
+      }
+  size_t count = 0;
+  for (size_t q = 0; q < blen; ++q) count += touched[q];
+  out->count = count;
+  out->idx = malloc(sizeof(uint32_t) * (count ? count : 1));
+  out->w = malloc(sizeof(double) * (count ? count : 1));
+  size_t e = 0;
+  for (size_t x = 0; x < bd[0]; ++x)
+    for (size_t y = 0; y < bd[1]; ++y)
+      for (size_t z = 0; z < bd[2]; ++z) {
+        const size_t cell = (x * bd[1] + y) * bd[2] + z;
+        if (!touched[cell]) continue;
+        out->idx[e] =
+            (uint32_t)(((lo[0] + x) * op->dims[1] + (lo[1] + y)) * op->dims[2] + (lo[2] + z));
+        out->w[e] = acc[cell];
+        ++e;
+      }
+  /* bounding box of the entries (grid.rs:202-214) */
+  for (int ax = 0; ax < 3; ++ax) {
+    out->lo[ax] = SIZE_MAX;
+    out->hi[ax] = 0;
+  }
+  for (size_t q = 0; q < count; ++q) {
+    const size_t g = out->idx[q];
+    const size_t pz = g % op->dims[2], rest = g / op->dims[2];
+    const size_t p[3] = {rest / op->dims[1], rest % op->dims[1], pz};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (p[ax] < out->lo[ax]) out->lo[ax] = p[ax];
+      if (p[ax] > out->hi[ax]) out->hi[ax] = p[ax];
+    }
+  }
+  free(acc);
+  free(touched);
+  for (int d = 0; d < 3; ++d) {
+    free(nx[d]);
+    free(nw[d]);
+  }
+  return FHO_OK;
+}
+
+/* neumann.rs bounds (near.rs:27-34) */
+static void bar_bounds(const bar *b, v3 *lo, v3 *hi) {
+  v3 reach = V(0, 0, 0);
+  for (int k = 0; k < 3; ++k) reach = vadd(reach, vmul(vabs(b->axes[k]), b->half[k]));
+  *lo = vsub(b->centre, reach);
+  *hi = vadd(b->centre, reach);
+}
+
+static int cmp_u32(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* near.rs:38-109: for each i, the j >= i within radius (and i), ascending. */
+static void near_pairs(const fho_filament *f, size_t n, double radius, size_t **off_out,
+                       uint32_t **idx_out) {
+  bar *bars = malloc(sizeof(bar) * n);
+  v3 *blo = malloc(sizeof(v3) * n), *bhi = malloc(sizeof(v3) * n);
+  for (size_t i = 0; i < n; ++i) {
+    bars[i] = bar_new(&f[i]);
+    bar_bounds(&bars[i], &blo[i], &bhi[i]);
+  }
+  v3 lo = blo[0], hi = bhi[0];
+  for (size_t i = 0; i < n; ++i) {
+    lo = V(fmin(lo.x, blo[i].x), fmin(lo.y, blo[i].y), fmin(lo.z, blo[i].z));
+    hi = V(fmax(hi.x, bhi[i].x), fmax(hi.y, bhi[i].y), fmax(hi.z, bhi[i].z));
+  }
+  const v3 extent = vsub(hi, lo);
+  const double e3[3] = {extent.x, extent.y, extent.z};
+  const double emax = fmax(fmax(extent.x, extent.y), extent.z);
+  double size = fmax(fmax(radius, emax * 1e-9), 2.2250738585072014e-308);
+  size_t dims[3];
+  for (;;) {
+    size_t prod = 1;
+    for (int a = 0; a < 3; ++a) {
+      dims[a] = (size_t)floor(e3[a] / size) + 1;
+      prod *= dims[a];
+    }
+    if (prod <= 8 * n + 64) break;
+    size *= 1.5;
+  }
+  const double l3[3] = {lo.x, lo.y, lo.z};
+#define CELL(x, a) \
+  ((size_t)fmin(fmax(floor(((x) - l3[a]) / size), 0.0), (double)(dims[a] - 1)))
+  const size_t nb = dims[0] * dims[1] * dims[2];
+  /* bins: counting pass, then filling in filament order */
+  size_t *bstart = calloc(nb + 1, sizeof(size_t));
+  for (size_t i = 0; i < n; ++i) {
+    const size_t f0[3] = {CELL(blo[i].x, 0), CELL(blo[i].y, 1), CELL(blo[i].z, 2)};
+    const size_t f1[3] = {CELL(bhi[i].x, 0), CELL(bhi[i].y, 1), CELL(bhi[i].z, 2)};
+    for (size_t a = f0[0]; a <= f1[0]; ++a)
+      for (size_t b = f0[1]; b <= f1[1]; ++b)
+        for (size_t c = f0[2]; c <= f1[2]; ++c) bstart[(a * dims[1] + b) * dims[2] + c + 1]++;
+  }
+  for (size_t q = 0; q < nb; ++q) bstart[q + 1] += bstart[q];
+  size_t *fillp = malloc(sizeof(size_t) * (nb ? nb : 1));
+  memcpy(fillp, bstart, sizeof(size_t) * nb);
+  uint32_t *items = malloc(sizeof(uint32_t) * (bstart[nb] ? bstart[nb] : 1));
+  for (size_t i = 0; i < n; ++i) {
+    const size_t f0[3] = {CELL(blo[i].x, 0), CELL(blo[i].y, 1), CELL(blo[i].z, 2)};
+    const size_t f1[3] = {CELL(bhi[i].x, 0), CELL(bhi[i].y, 1), CELL(bhi[i].z, 2)};
+    for (size_t a = f0[0]; a <= f1[0]; ++a)
+      for (size_t b = f0[1]; b <= f1[1]; ++b)
+        for (size_t c = f0[2]; c <= f1[2]; ++c)
+          items[fillp[(a * dims[1] + b) * dims[2] + c]++] = (uint32_t)i;
+  }
+  uint32_t **rows = malloc(sizeof(uint32_t *) * (n ? n : 1));
+  size_t *rlen = malloc(sizeof(size_t) * (n ? n : 1));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    const size_t i = (size_t)ii;
+    const v3 a0 = vsub(blo[i], V(radius, radius, radius));
+    const v3 a1 = vadd(bhi[i], V(radius, radius, radius));
+    const size_t f0[3] = {CELL(a0.x, 0), CELL(a0.y, 1), CELL(a0.z, 2)};
+    const size_t f1[3] = {CELL(a1.x, 0), CELL(a1.y, 1), CELL(a1.z, 2)};
+    size_t cap = 64, len = 0;
+    uint32_t *found = malloc(sizeof(uint32_t) * cap);
+    for (size_t a = f0[0]; a <= f1[0]; ++a)
+      for (size_t b = f0[1]; b <= f1[1]; ++b)
+        for (size_t c = f0[2]; c <= f1[2]; ++c) {
+          const size_t q = (a * dims[1] + b) * dims[2] + c;
+          for (size_t e = bstart[q]; e < bstart[q + 1]; ++e) {
+            if (items[e] < i) continue;
+            if (len == cap) {
+              cap *= 2;
+              found = realloc(found, sizeof(uint32_t) * cap);
+            }
+            found[len++] = items[e];
+          }
+        }
+    qsort(found, len, sizeof(uint32_t), cmp_u32);
+    size_t w = 0;
+    for (size_t e = 0; e < len; ++e) {
+      if (e > 0 && found[e] == found[e - 1]) continue; /* dedup on the sorted input */
+      const uint32_t other = found[e];
+      if (other == i || separation(&bars[i], &bars[other]) < radius) found[w++] = other;
+    }
+    rows[i] = found;
+    rlen[i] = w;
+  }
+#undef CELL
+  size_t *off = malloc(sizeof(size_t) * (n + 1));
+  off[0] = 0;
+  for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + rlen[i];
+  uint32_t *idx = malloc(sizeof(uint32_t) * (off[n] ? off[n] : 1));
+  for (size_t i = 0; i < n; ++i) {
+    memcpy(idx + off[i], rows[i], sizeof(uint32_t) * rlen[i]);
+    free(rows[i]);
+  }
+  free(rows);
+  free(rlen);
+  free(fillp);
+  free(items);
+  free(bstart);
+  free(bars);
+  free(blo);
+  free(bhi);
+  *off_out = off;
+  *idx_out = idx;
+}
+
+/* near.rs:131-139 */
+static double grid_kernel(int64_t dx, int64_t dy, int64_t dz, double h) {
+  const double r2 = (double)(dx * dx + dy * dy + dz * dz);
+  return r2 == 0.0 ? 0.0 : 1.0 / (h * sqrt(r2));
+}
+
+/* near.rs:316-389: sum_g sum_h W_ig W_jh K(g - h) for every partner j. */
+static void grid_interactions(const struct fho_pfft *op, const projection *P, size_t i,
+                              const size_t *partners, size_t np, const int64_t reach[3],
+                              const size_t tdims[3], const double *table, double *out) {
+  const projection *src = &P[i];
+  size_t lo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, hi[3] = {0, 0, 0};
+  size_t partner_entries = 0;
+  for (size_t q = 0; q < np; ++q) {
+    const projection *pj = &P[partners[q]];
+    for (int a = 0; a < 3; ++a) {
+      if (pj->lo[a] < lo[a]) lo[a] = pj->lo[a];
+      if (pj->hi[a] > hi[a]) hi[a] = pj->hi[a];
+    }
+    partner_entries += pj->count;
+  }
+  const size_t bd[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  const size_t box_len = bd[0] * bd[1] * bd[2];
+  const size_t direct_cost = src->count * partner_entries;
+  const size_t box_cost = src->count * box_len + partner_entries;
+  const size_t d1 = op->dims[1], d2 = op->dims[2];
+#define PT(g, p)                                   \
+  do {                                             \
+    const size_t z_ = (g) % d2, r_ = (g) / d2;     \
+    (p)[0] = (int64_t)(r_ / d1);                   \
+    (p)[1] = (int64_t)(r_ % d1);                   \
+    (p)[2] = (int64_t)z_;                          \
+  } while (0)
+#define TIDX(d0, d1_, d2_) \
+  ((size_t)(((d0) + reach[0]) * (int64_t)tdims[1] + ((d1_) + reach[1])) * tdims[2] + (size_t)((d2_) + reach[2]))
+  if (direct_cost <= box_cost) {
+    for (size_t q = 0; q < np; ++q) {
+      const projection *pj = &P[partners[q]];
+      double total = 0.0;
+      for (size_t e = 0; e < pj->count; ++e) {
+        int64_t hp[3];
+        PT(pj->idx[e], hp);
+        double potential = 0.0;
+        for (size_t s = 0; s < src->count; ++s) {
+          int64_t gp[3];
+          PT(src->idx[s], gp);
+          potential += src->w[s] * table[TIDX(hp[0] - gp[0], hp[1] - gp[1], hp[2] - gp[2])];
+        }
+        total += pj->w[e] * potential;
+      }
+      out[q] = total;
+    }
+    return;
+  }
+  double *psi = calloc(box_len, sizeof(double));
+  const size_t bx = bd[0], by = bd[1], bz = bd[2];
+  for (size_t s = 0; s < src->count; ++s) {
+    int64_t g[3];
+    PT(src->idx[s], g);
+    const double wg = src->w[s];
+    for (size_t x = 0; x < bx; ++x)
+      for (size_t y = 0; y < by; ++y) {
+        const double *kernel =
+            &table[TIDX((int64_t)(lo[0] + x) - g[0], (int64_t)(lo[1] + y) - g[1], (int64_t)lo[2] - g[2])];
+        double *row = &psi[(x * by + y) * bz];
+        /* near.rs:189-207: fused multiply-add in blocks of four, plain
+         * multiply-add for the tail */
+        const size_t body = bz - bz % 4;
+        for (size_t k = 0; k < body; ++k) row[k] = fma(wg, kernel[k], row[k]);
+        for (size_t k = body; k < bz; ++k) row[k] += wg * kernel[k];
+      }
+  }
+  for (size_t q = 0; q < np; ++q) {
+    const projection *pj = &P[partners[q]];
+    double total = 0.0;
+    for (size_t e = 0; e < pj->count; ++e) {
+      int64_t p[3];
+      PT(pj->idx[e], p);
+      const size_t local = (((size_t)p[0] - lo[0]) * by + ((size_t)p[1] - lo[1])) * bz + ((size_t)p[2] - lo[2]);
+      total += pj->w[e] * psi[local];
+    }
+    out[q] = total;
+  }
+  free(psi);
+#undef PT
+#undef TIDX
+}
+
+/* ---- FFT (mixed radix 2, 3, 5; plain DFT for any other factor) ---- */
+static void fft1(cplx *x, cplx *work, size_t n, size_t stride, int inverse, const cplx *tw,
+                 size_t tw_n) {
+  /* x: n values at stride; result written back in place (stride). */
+  if (n == 1) return;
+  size_t p = 2;
+  if (n % 2) p = n % 3 == 0 ? 3 : (n % 5 == 0 ? 5 : n);
+  const size_t q = n / p;
+  /* gather the p decimated sub-sequences contiguously into work, transform each */
+  for (size_t r = 0; r < p; ++r)
+    for (size_t k = 0; k < q; ++k) work[r * q + k] = x[(k * p + r) * stride];
+  for (size_t r = 0; r < p; ++r) fft1(work + r * q, work + n, q, 1, inverse, tw, tw_n);
+  const size_t step = tw_n / n;
+  for (size_t k = 0; k < q; ++k)
+    for (size_t s = 0; s < p; ++s) {
+      const size_t out = k + s * q;
+      cplx acc = {0.0, 0.0};
+      for (size_t r = 0; r < p; ++r) {
+        const size_t e = (r * out) % n;
+        cplx w = tw[e * step];
+        if (inverse) w.im = -w.im;
+        const cplx v = cmul(w, work[r * q + k]);
+        acc.re += v.re;
+        acc.im += v.im;
+      }
+      x[out * stride] = acc;
+    }
+}
+
+/* in-place 1-D transform of every line along `axis` of a d0 x d1 x d2 array */
+static void fft_axis(cplx *data, const size_t d[3], int axis, int inverse) {
+  const size_t n = d[axis];
+  if (n <= 1) return;
+  cplx *tw = malloc(sizeof(cplx) * n);
+  for (size_t k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * (double)k / (double)n;
+    tw[k].re = cos(a);
+    tw[k].im = sin(a);
+  }
+  const size_t stride = axis == 2 ? 1 : (axis == 1 ? d[2] : d[1] * d[2]);
+  const size_t lines = d[0] * d[1] * d[2] / n;
+#pragma omp parallel
+  {
+    cplx *line = malloc(sizeof(cplx) * n);
+    cplx *work = malloc(sizeof(cplx) * 4 * n);
+#pragma omp for schedule(static)
+    for (int64_t li = 0; li < (int64_t)lines; ++li) {
+      size_t base;
+      if (axis == 2) base = (size_t)li * n;
+      else if (axis == 1) base = ((size_t)li / d[2]) * d[1] * d[2] + (size_t)li % d[2];
+      else base = (size_t)li;
+      for (size_t k = 0; k < n; ++k) line[k] = data[base + k * stride];
+      fft1(line, work, n, 1, inverse, tw, n);
+      for (size_t k = 0; k < n; ++k) data[base + k * stride] = line[k];
+    }
+    free(line);
+    free(work);
+  }
+  free(tw);
+}
+
+static void fft3(cplx *data, const size_t d[3], int inverse) {
+  if (!inverse) {
+    fft_axis(data, d, 2, 0);
+    fft_axis(data, d, 1, 0);
+    fft_axis(data, d, 0, 0);
+  } else {
+    fft_axis(data, d, 0, 1);
+    fft_axis(data, d, 1, 1);
+    fft_axis(data, d, 2, 1);
+  }
+}
+
+/* fft.rs:91-140: the kernel's spectrum on the padded grid, wrapped offsets. */
+static void build_spectrum(struct fho_pfft *op) {
+  const size_t *N = op->dims, *M = op->padded;
+  const size_t total = M[0] * M[1] * M[2];
+  cplx *data = calloc(total, sizeof(cplx));
+  for (size_t i = 0; i < M[0]; ++i) {
+    int64_t di;
+    if (i < N[0]) di = (int64_t)i;
+    else if (i + N[0] > M[0]) di = (int64_t)i - (int64_t)M[0];
+    else continue;
+    for (size_t j = 0; j < M[1]; ++j) {
+      int64_t dj;
+      if (j < N[1]) dj = (int64_t)j;
+      else if (j + N[1] > M[1]) dj = (int64_t)j - (int64_t)M[1];
+      else continue;
+      for (size_t k = 0; k < M[2]; ++k) {
+        int64_t dk;
+        if (k < N[2]) dk = (int64_t)k;
+        else if (k + N[2] > M[2]) dk = (int64_t)k - (int64_t)M[2];
+        else continue;
+        data[(i * M[1] + j) * M[2] + k].re = grid_kernel(di, dj, dk, op->h);
+      }
+    }
+  }
+  fft3(data, M, 0);
+  const double scale = 1.0 / (double)total;
+  op->spectrum = malloc(sizeof(double) * total);
+  for (size_t q = 0; q < total; ++q) op->spectrum[q] = data[q].re * scale;
+  free(data);
+}
+
+/* fft.rs:155-220: phi_a, phi_b = K * a, K * b (b may be NULL). */
+static void convolve(const struct fho_pfft *op, const double *a, const double *b, double *pa,
+                     double *pb) {
+  const size_t *N = op->dims, *M = op->padded;
+  const size_t total = M[0] * M[1] * M[2];
+  cplx *data = calloc(total, sizeof(cplx));
+  for (size_t i = 0; i < N[0]; ++i)
+    for (size_t j = 0; j < N[1]; ++j)
+      for (size_t k = 0; k < N[2]; ++k) {
+        const size_t s = (i * N[1] + j) * N[2] + k;
+        cplx *d = &data[(i * M[1] + j) * M[2] + k];
+        d->re = a[s];
+        d->im = b ? b[s] : 0.0;
+      }
+  fft3(data, M, 0);
+  for (size_t q = 0; q < total; ++q) {
+    data[q].re *= op->spectrum[q];
+    data[q].im *= op->spectrum[q];
+  }
+  fft3(data, M, 1);
+  for (size_t i = 0; i < N[0]; ++i)
+    for (size_t j = 0; j < N[1]; ++j)
+      for (size_t k = 0; k < N[2]; ++k) {
+        const size_t s = (i * N[1] + j) * N[2] + k;
+        const cplx d = data[(i * M[1] + j) * M[2] + k];
+        pa[s] = d.re;
+        if (pb) pb[s] = d.im;
+      }
+  free(data);
+}
+
+int fho_pfft_convolve(const fho_pfft *op, const double *a, const double *b, double *pa,
+                      double *pb) {
+  if (!op->has_grid) return FHO_E_INVALID_ARGUMENT;
+  convolve(op, a, b, pa, pb);
+  return FHO_OK;
+}
+
+void fho_pfft_destroy(fho_pfft *op) {
+  if (!op) return;
+  free(op->dirs);
+  free(op->proj_off);
+  free(op->proj_idx);
+  free(op->proj_w);
+  free(op->spread_off);
+  free(op->spread_fil);
+  free(op->spread_w);
+  free(op->near_off);
+  free(op->near_col);
+  free(op->near_val);
+  free(op->spectrum);
+  free(op);
+}
+
+/* mod.rs:332-445 */
+int fho_pfft_create(const fho_filament *f, size_t n, const fho_pfft_params *prm, fho_pfft **out) {
+  ensure_tables();
+  *out = NULL;
+  const int order = prm->interpolation_order;
+  if (order < 1) return fail(FHO_E_INVALID_ARGUMENT, 1, 0, 0, 0.0);
+  if (!(isfinite(prm->near_field_radius) && prm->near_field_radius >= (double)(order + 1)))
+    return fail(FHO_E_INVALID_ARGUMENT, 2, 0, 0, 0.0);
+  if (prm->auto_grid) {
+    if (!(prm->cells_per_filament > 0.0 && isfinite(prm->cells_per_filament)))
+      return fail(FHO_E_INVALID_ARGUMENT, 3, 0, 0, 0.0);
+  } else if (!(prm->spacing > 0.0 && isfinite(prm->spacing))) {
+    return fail(FHO_E_INVALID_ARGUMENT, 4, 0, 0, 0.0);
+  }
+  struct fho_pfft *op = calloc(1, sizeof(struct fho_pfft));
+  op->n = n;
+  op->m = order + 1;
+  op->proj_off = calloc(n + 1, sizeof(size_t));
+  op->near_off = calloc(n + 1, sizeof(size_t));
+  op->dirs = malloc(sizeof(double) * 3 * (n ? n : 1));
+  if (n == 0) {
+    *out = op;
+    return FHO_OK;
+  }
+  /* mod.rs:580-593 bounding box */
+  v3 lo = V(INFINITY, INFINITY, INFINITY), hi = V(-INFINITY, -INFINITY, -INFINITY);
+  for (size_t i = 0; i < n; ++i) {
+    const v3 reach =
+        vmul(vadd(vmul(vabs(vload(f[i].wdir)), f[i].width), vmul(vabs(vload(f[i].hdir)), f[i].height)), 0.5);
+    const v3 ends[2] = {vload(f[i].start), vload(f[i].end)};
+    for (int e = 0; e < 2; ++e) {
+      const v3 a = vsub(ends[e], reach), b = vadd(ends[e], reach);
+      lo = V(fmin(lo.x, a.x), fmin(lo.y, a.y), fmin(lo.z, a.z));
+      hi = V(fmax(hi.x, b.x), fmax(hi.y, b.y), fmax(hi.z, b.z));
+    }
+  }
+  const double ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+  const double h = prm->auto_grid ? fho_auto_spacing(ext, prm->cells_per_filament * (double)n)
+                                  : prm->spacing;
+  /* grid.rs:64-80 */
+  const double pad = (double)((op->m + 1) / 2);
+  const double lo3[3] = {lo.x, lo.y, lo.z}, hi3[3] = {hi.x, hi.y, hi.z};
+  size_t padded_total = 1, grid_total = 1;
+  for (int a = 0; a < 3; ++a) {
+    op->origin[a] = lo3[a] - pad * h;
+    const double span = (hi3[a] - op->origin[a]) / h;
+    op->dims[a] = (size_t)floor(span + 0.5 * (double)op->m) + 2;
+    op->padded[a] = fho_smooth_size(2 * op->dims[a] - 1);
+    padded_total *= op->padded[a];
+    grid_total *= op->dims[a];
+  }
+  op->h = h;
+  if (padded_total > PFFT_MAX_FFT_POINTS || grid_total > UINT32_MAX) {
+    fho_pfft_destroy(op);
+    return fail(FHO_E_INVALID_ARGUMENT, 5, (int64_t)padded_total, (int64_t)PFFT_MAX_FFT_POINTS, 0.0);
+  }
+  op->has_grid = 1;
+  const double radius = prm->near_field_radius * h;
+  /* projections */
+  double thresholds[PFFT_QUAD_MAX];
+  table_new(PFFT_QUAD_TOL, PFFT_QUAD_MAX, thresholds);
+  projection *P = calloc(n, sizeof(projection));
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t i = 0; i < (int64_t)n; ++i) project_one(op, &f[i], radius, thresholds, &P[i]);
+  for (size_t i = 0; i < n; ++i) op->proj_off[i + 1] = op->proj_off[i] + P[i].count;
+  const size_t E = op->proj_off[n];
+  op->proj_idx = malloc(sizeof(uint32_t) * (E ? E : 1));
+  op->proj_w = malloc(sizeof(double) * (E ? E : 1));
+  for (size_t i = 0; i < n; ++i) {
+    memcpy(op->proj_idx + op->proj_off[i], P[i].idx, sizeof(uint32_t) * P[i].count);
+    memcpy(op->proj_w + op->proj_off[i], P[i].w, sizeof(double) * P[i].count);
+  }
+  /* transpose (mod.rs:395-417) */
+  op->spread_off = calloc(grid_total + 1, sizeof(size_t));
+  for (size_t e = 0; e < E; ++e) op->spread_off[op->proj_idx[e] + 1]++;
+  for (size_t g = 0; g < grid_total; ++g) op->spread_off[g + 1] += op->spread_off[g];
+  size_t *fillp = malloc(sizeof(size_t) * grid_total);
+  memcpy(fillp, op->spread_off, sizeof(size_t) * grid_total);
+  op->spread_fil = malloc(sizeof(uint32_t) * (E ? E : 1));
+  op->spread_w = malloc(sizeof(double) * (E ? E : 1));
+  for (size_t i = 0; i < n; ++i)
+    for (size_t e = op->proj_off[i]; e < op->proj_off[i + 1]; ++e) {
+      const size_t slot = fillp[op->proj_idx[e]]++;
+      op->spread_fil[slot] = (uint32_t)i;
+      op->spread_w[slot] = op->proj_w[e];
+    }
+  free(fillp);
+  for (size_t i = 0; i < n; ++i) {
+    op->dirs[3 * i] = f[i].dir[0];
+    op->dirs[3 * i + 1] = f[i].dir[1];
+    op->dirs[3 * i + 2] = f[i].dir[2];
+  }
+  op->ncomp = 0;
+  for (int c = 0; c < 3; ++c) {
+    int any = 0;
+    for (size_t i = 0; i < n && !any; ++i) any = op->dirs[3 * i + c] != 0.0;
+    if (any) op->comps[op->ncomp++] = c;
+  }
+  /* near pairs and precorrection (near.rs:213-313) */
+  size_t *noff;
+  uint32_t *nidx;
+  near_pairs(f, n, radius, &noff, &nidx);
+  size_t **partners = malloc(sizeof(size_t *) * n);
+  size_t *np = calloc(n, sizeof(size_t));
+  for (size_t i = 0; i < n; ++i) {
+    partners[i] = malloc(sizeof(size_t) * (noff[i + 1] - noff[i] + 1));
+    const v3 di = vload(f[i].dir);
+    for (size_t e = noff[i]; e < noff[i + 1]; ++e) {
+      const size_t j = nidx[e];
+      if (fabs(vdot(di, vload(f[j].dir))) >= 1e-14) partners[i][np[i]++] = j;
+    }
+  }
+  int64_t reach[3] = {0, 0, 0};
+  for (size_t i = 0; i < n; ++i) {
+    if (!np[i]) continue;
+    size_t plo[3] = {SIZE_MAX, SIZE_MAX, SIZE_MAX}, phi_[3] = {0, 0, 0};
+    for (size_t q = 0; q < np[i]; ++q)
+      for (int a = 0; a < 3; ++a) {
+        if (P[partners[i][q]].lo[a] < plo[a]) plo[a] = P[partners[i][q]].lo[a];
+        if (P[partners[i][q]].hi[a] > phi_[a]) phi_[a] = P[partners[i][q]].hi[a];
+      }
+    for (int a = 0; a < 3; ++a) {
+      const size_t own_lo = P[i].lo[a], own_hi = P[i].hi[a];
+      const size_t r1 = phi_[a] - (own_lo < phi_[a] ? own_lo : phi_[a]);
+      const size_t r2 = own_hi - (plo[a] < own_hi ? plo[a] : own_hi);
+      const int64_t r = (int64_t)(r1 > r2 ? r1 : r2);
+      if (r > reach[a]) reach[a] = r;
+    }
+  }
+  const size_t tdims[3] = {2 * (size_t)reach[0] + 1, 2 * (size_t)reach[1] + 1, 2 * (size_t)reach[2] + 1};
+  double *table = malloc(sizeof(double) * tdims[0] * tdims[1] * tdims[2]);
+  for (size_t x = 0; x < tdims[0]; ++x)
+    for (size_t y = 0; y < tdims[1]; ++y)
+      for (size_t z = 0; z < tdims[2]; ++z)
+        table[(x * tdims[1] + y) * tdims[2] + z] =
+            grid_kernel((int64_t)x - reach[0], (int64_t)y - reach[1], (int64_t)z - reach[2], h);
+  double **upper = malloc(sizeof(double *) * n);
+  int status = FHO_OK;
+  int64_t err_i = -1, err_j = -1;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    const size_t i = (size_t)ii;
+    upper[i] = malloc(sizeof(double) * (np[i] ? np[i] : 1));
+    if (!np[i]) continue;
+    double *approx = malloc(sizeof(double) * np[i]);
+    grid_interactions(op, P, i, partners[i], np[i], reach, tdims, table, approx);
+    const v3 di = vload(f[i].dir);
+    for (size_t q = 0; q < np[i]; ++q) {
+      const size_t j = partners[i][q];
+      double exact;
+      int meth, res;
+      if (fho_mutual(&f[i], &f[j], &exact, &meth, &res) != FHO_OK) {
+#pragma omp critical
+        {
+          if (err_i < 0 || (int64_t)i < err_i || ((int64_t)i == err_i && (int64_t)j < err_j)) {
+            err_i = (int64_t)i;
+            err_j = (int64_t)j;
+            status = FHO_E_KERNEL;
+          }
+        }
+        exact = NAN;
+      }
+      const double cos_ = vdot(di, vload(f[j].dir));
+      upper[i][q] = exact - 1.0e-7 * cos_ * approx[q];
+    }
+    free(approx);
+  }
+  free(table);
+  if (status) {
+    for (size_t i = 0; i < n; ++i) {
+      free(upper[i]);
+      free(partners[i]);
+      free(P[i].idx);
+      free(P[i].w);
+    }
+    free(upper);
+    free(partners);
+    free(np);
+    free(P);
+    free(noff);
+    free(nidx);
+    fho_pfft_destroy(op);
+    return fail(FHO_E_KERNEL, 2, err_i, err_j, 0.0);
+  }
+  /* mirror into full rows (near.rs:264-300) */
+  size_t *cnt = calloc(n, sizeof(size_t));
+  for (size_t i = 0; i < n; ++i)
+    for (size_t q = 0; q < np[i]; ++q) {
+      cnt[i]++;
+      if (partners[i][q] != i) cnt[partners[i][q]]++;
+    }
+  for (size_t i = 0; i < n; ++i) op->near_off[i + 1] = op->near_off[i] + cnt[i];
+  const size_t NE = op->near_off[n];
+  op->near_col = malloc(sizeof(uint32_t) * (NE ? NE : 1));
+  op->near_val = malloc(sizeof(double) * (NE ? NE : 1));
+  memcpy(cnt, op->near_off, sizeof(size_t) * n);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t q = 0; q < np[i]; ++q) {
+      const size_t j = partners[i][q];
+      op->near_col[cnt[i]] = (uint32_t)j;
+      op->near_val[cnt[i]++] = upper[i][q];
+      if (j != i) {
+        op->near_col[cnt[j]] = (uint32_t)i;
+        op->near_val[cnt[j]++] = upper[i][q];
+      }
+    }
+  free(cnt);
+  for (size_t i = 0; i < n; ++i) {
+    free(upper[i]);
+    free(partners[i]);
+    free(P[i].idx);
+    free(P[i].w);
+  }
+  free(upper);
+  free(partners);
+  free(np);
+  free(P);
+  free(noff);
+  free(nidx);
+  build_spectrum(op);
+  *out = op;
+  return FHO_OK;
+}
+
+/* mod.rs:482-528 */
+void fho_pfft_apply(const fho_pfft *op, const double *x, double *y) {
+  const size_t n = op->n;
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    double s = 0.0;
+    for (size_t e = op->near_off[ii]; e < op->near_off[ii + 1]; ++e)
+      s += op->near_val[e] * x[op->near_col[e]];
+    y[ii] = s;
+  }
+  if (!op->has_grid) return;
+  const size_t G = op->dims[0] * op->dims[1] * op->dims[2];
+  double *src[3] = {NULL, NULL, NULL}, *pot[3] = {NULL, NULL, NULL};
+  for (int q = 0; q < op->ncomp; ++q) {
+    const int c = op->comps[q];
+    src[q] = malloc(sizeof(double) * G);
+    pot[q] = malloc(sizeof(double) * G);
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < (int64_t)G; ++g) {
+      double s = 0.0;
+      for (size_t e = op->spread_off[g]; e < op->spread_off[g + 1]; ++e) {
+        const size_t i = op->spread_fil[e];
+        s += op->spread_w[e] * op->dirs[3 * i + c] * x[i];
+      }
+      src[q][g] = s;
+    }
+  }
+  for (int q = 0; q < op->ncomp; q += 2)
+    convolve(op, src[q], q + 1 < op->ncomp ? src[q + 1] : NULL, pot[q],
+             q + 1 < op->ncomp ? pot[q + 1] : NULL);
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
+    double far = 0.0;
+    for (int q = 0; q < op->ncomp; ++q) {
+      double sum = 0.0;
+      for (size_t e = op->proj_off[ii]; e < op->proj_off[ii + 1]; ++e)
+        sum += op->proj_w[e] * pot[q][op->proj_idx[e]];
+      far += op->dirs[3 * ii + op->comps[q]] * sum;
+    }
+    y[ii] += 1.0e-7 * far;
+  }
+  for (int q = 0; q < op->ncomp; ++q) {
+    free(src[q]);
+    free(pot[q]);
+  }
+}
+
+void fho_pfft_stats(const fho_pfft *op, double *spacing, size_t dims[3], size_t padded[3],
+                    size_t *projection_entries, size_t *near_entries) {
+  *spacing = op->h;
+  for (int a = 0; a < 3; ++a) {
+    dims[a] = op->has_grid ? op->dims[a] : 0;
+    padded[a] = op->has_grid ? op->padded[a] : 0;
+  }
+  *projection_entries = op->proj_off[op->n];
+  *near_entries = op->near_off[op->n];
+}
+
+int fho_pfft_data(const fho_pfft *op, size_t *proj_off, uint32_t *proj_idx, double *proj_w,
+                  size_t *near_off, uint32_t *near_col, double *near_val, double origin[3]) {
+  if (proj_off) memcpy(proj_off, op->proj_off, sizeof(size_t) * (op->n + 1));
+  if (proj_idx) memcpy(proj_idx, op->proj_idx, sizeof(uint32_t) * op->proj_off[op->n]);
+  if (proj_w) memcpy(proj_w, op->proj_w, sizeof(double) * op->proj_off[op->n]);
+  if (near_off) memcpy(near_off, op->near_off, sizeof(size_t) * (op->n + 1));
+  if (near_col) memcpy(near_col, op->near_col, sizeof(uint32_t) * op->near_off[op->n]);
+  if (near_val) memcpy(near_val, op->near_val, sizeof(double) * op->near_off[op->n]);
+  if (origin)
+    for (int a = 0; a < 3; ++a) origin[a] = op->origin[a];
+  return FHO_OK;
+}
+
+/* ======================================================================== */
+/* gmres.rs: restarted, right-preconditioned GMRES (MGS twice, Givens)      */
+/* ======================================================================== */
+
+/* Complex and real forms of gmres.rs:91-206 with its helpers (:209-257);
+ * the real form is the same recurrence with every imaginary part zero. */
+typedef struct {
+  double re, im;
+} gz;
+static inline gz gz_add(gz a, gz b) { gz r = {a.re + b.re, a.im + b.im}; return r; }
+static inline gz gz_sub(gz a, gz b) { gz r = {a.re - b.re, a.im - b.im}; return r; }
+static inline gz gz_mul(gz a, gz b) {
+  gz r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+static inline gz gz_conj(gz a) { gz r = {a.re, -a.im}; return r; }
+static inline gz gz_scale(gz a, double s) { gz r = {a.re * s, a.im * s}; return r; }
+static inline gz gz_unscale(gz a, double s) { gz r = {a.re / s, a.im / s}; return r; }
+static inline double gz_mod(gz a) { return hypot(a.re, a.im); }
+static inline gz gz_div(gz a, gz b) { /* num-complex Div */
+  const double n = b.re * b.re + b.im * b.im;
+  gz r = {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+  return r;
+}
+static inline gz gz_neg(gz a) { gz r = {-a.re, -a.im}; return r; }
+
+static double gz_norm(const gz *v, size_t n, int complex_) {
+  double scale = 0.0;
+  for (size_t i = 0; i < n; ++i) scale = fmax(scale, complex_ ? gz_mod(v[i]) : fabs(v[i].re));
+  if (scale == 0.0 || !isfinite(scale)) return scale;
+  double sum = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const gz u = gz_unscale(v[i], scale);
+    sum += complex_ ? u.re * u.re + u.im * u.im : u.re * u.re;
+  }
+  return scale * sqrt(sum);
+}
+
+static gz gz_dot(const gz *u, const gz *v, size_t n) {
+  gz s = {0.0, 0.0};
+  for (size_t i = 0; i < n; ++i) s = gz_add(s, gz_mul(gz_conj(u[i]), v[i]));
+  return s;
+}
+
+static void gz_axpy(gz a, const gz *x, gz *y, size_t n) {
+  for (size_t i = 0; i < n; ++i) y[i] = gz_add(y[i], gz_mul(a, x[i]));
+}
+
+/* gmres.rs:246-257 */
+static void givens(gz a, gz b, int complex_, double *c, gz *s, gz *r) {
+  const double abs_a = complex_ ? gz_mod(a) : fabs(a.re), abs_b = complex_ ? gz_mod(b) : fabs(b.re);
+  if (abs_b == 0.0) {
+    *c = 1.0;
+    s->re = s->im = 0.0;
+    *r = a;
+    return;
+  }
+  if (abs_a == 0.0) {
+    *c = 0.0;
+    s->re = 1.0;
+    s->im = 0.0;
+    *r = b;
+    return;
+  }
+  const double length = hypot(abs_a, abs_b);
+  const gz phase = gz_unscale(a, abs_a);
+  *c = abs_a / length;
+  *s = gz_mul(phase, gz_unscale(gz_conj(b), length));
+  *r = gz_scale(phase, length);
+}
+
+typedef void (*gz_op)(void *ctx, const gz *in, gz *out);
+
+typedef struct {
+  size_t iterations;
+  double relative_residual;
+  int converged;
+} gz_outcome;
+
+static void gz_residual(gz_op apply, void *actx, const gz *b, const gz *x, gz *r, size_t n) {
+  apply(actx, x, r);
+  for (size_t i = 0; i < n; ++i) r[i] = gz_sub(b[i], r[i]);
+}
+
+static gz_outcome gmres_core(gz_op apply, void *actx, gz_op precondition, void *pctx,
+                             const gz *b, gz *x, size_t n, double tolerance, size_t restart_in,
+                             size_t max_iterations, int complex_) {
+  gz_outcome out = {0, 0.0, 1};
+  const double b_norm = gz_norm(b, n, complex_);
+  if (b_norm == 0.0) {
+    for (size_t i = 0; i < n; ++i) x[i].re = x[i].im = 0.0;
+    return out;
+  }
+  const double target = tolerance * b_norm;
+  size_t restart = restart_in < 1 ? 1 : restart_in;
+  if (restart > n) restart = n;
+  gz *residual = malloc(sizeof(gz) * n);
+  gz *basis = malloc(sizeof(gz) * n * (restart + 1));
+  gz *w = malloc(sizeof(gz) * n), *tmp = malloc(sizeof(gz) * n), *update = malloc(sizeof(gz) * n);
+  gz *hess = malloc(sizeof(gz) * (restart + 1) * restart); /* column j: hess[j*(restart+1) + i] */
+  double *rc = malloc(sizeof(double) * restart);
+  gz *rs = malloc(sizeof(gz) * restart), *g = malloc(sizeof(gz) * (restart + 1));
+  gz *y = malloc(sizeof(gz) * restart);
+  size_t iterations = 0;
+  gz_residual(apply, actx, b, x, residual, n);
+  double r_norm = gz_norm(residual, n, complex_);
+  for (;;) {
+    if (r_norm <= target) {
+      out.converged = 1;
+      break;
+    }
+    if (iterations >= max_iterations || !isfinite(r_norm)) {
+      out.converged = 0;
+      break;
+    }
+    for (size_t i = 0; i < n; ++i) basis[i] = gz_unscale(residual[i], r_norm);
+    size_t steps = 0, nb = 1;
+    for (size_t i = 0; i <= restart; ++i) g[i].re = g[i].im = 0.0;
+    g[0].re = r_norm;
+    while (steps < restart && iterations < max_iterations) {
+      const size_t j = steps;
+      precondition(pctx, basis + j * n, tmp);
+      apply(actx, tmp, w);
+      ++iterations;
+      gz *column = hess + j * (restart + 1);
+      for (size_t i = 0; i < j + 2; ++i) column[i].re = column[i].im = 0.0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (size_t v = 0; v < nb; ++v) {
+          const gz projection = gz_dot(basis + v * n, w, n);
+          column[v] = gz_add(column[v], projection);
+          gz_axpy(gz_neg(projection), basis + v * n, w, n);
+        }
+      const double next_norm = gz_norm(w, n, complex_);
+      column[j + 1].re = next_norm;
+      column[j + 1].im = 0.0;
+      for (size_t i = 0; i < j; ++i) {
+        const gz a = column[i], bb = column[i + 1];
+        column[i] = gz_add(gz_scale(a, rc[i]), gz_mul(rs[i], bb));
+        column[i + 1] = gz_sub(gz_scale(bb, rc[i]), gz_mul(gz_conj(rs[i]), a));
+      }
+      double c;
+      gz s, r;
+      givens(column[j], column[j + 1], complex_, &c, &s, &r);
+      column[j] = r;
+      column[j + 1].re = column[j + 1].im = 0.0;
+      rc[j] = c;
+      rs[j] = s;
+      const gz gj = g[j];
+      g[j] = gz_scale(gj, c);
+      g[j + 1] = gz_neg(gz_mul(gz_conj(s), gj));
+      ++steps;
+      if (next_norm == 0.0 || (complex_ ? gz_mod(g[j + 1]) : fabs(g[j + 1].re)) <= target) break;
+      for (size_t i = 0; i < n; ++i) basis[nb * n + i] = gz_unscale(w[i], next_norm);
+      ++nb;
+    }
+    for (size_t ii = steps; ii-- > 0;) {
+      gz sum = g[ii];
+      for (size_t k = ii + 1; k < steps; ++k)
+        sum = gz_sub(sum, gz_mul(hess[k * (restart + 1) + ii], y[k]));
+      y[ii] = gz_div(sum, hess[ii * (restart + 1) + ii]);
+    }
+    for (size_t i = 0; i < n; ++i) update[i].re = update[i].im = 0.0;
+    for (size_t k = 0; k < steps; ++k) gz_axpy(y[k], basis + k * n, update, n);
+    precondition(pctx, update, tmp);
+    for (size_t i = 0; i < n; ++i) x[i] = gz_add(x[i], tmp[i]);
+    gz_residual(apply, actx, b, x, residual, n);
+    r_norm = gz_norm(residual, n, complex_);
+  }
+  out.iterations = iterations;
+  out.relative_residual = r_norm / b_norm;
+  free(residual);
+  free(basis);
+  free(w);
+  free(tmp);
+  free(update);
+  free(hess);
+  free(rc);
+  free(rs);
+  free(g);
+  free(y);
+  return out;
+}
+
+typedef struct {
+  const gz *a;
+  size_t n;
+} dense_ctx;
+
+static void dense_apply(void *ctx, const gz *in, gz *out) {
+  const dense_ctx *d = ctx;
+  for (size_t i = 0; i < d->n; ++i) {
+    gz s = {0.0, 0.0};
+    for (size_t k = 0; k < d->n; ++k) s = gz_add(s, gz_mul(d->a[k * d->n + i], in[k]));
+    out[i] = s;
+  }
+}
+static void identity_op(void *ctx, const gz *in, gz *out) {
+  const size_t n = *(const size_t *)ctx;
+  memcpy(out, in, sizeof(gz) * n);
+}
+
+/* GMRES on a dense column-major complex matrix (interleaved {re, im}), no
+ * preconditioner: for checking gmres.rs against a direct solve. */
+int fho_gmres_dense(size_t n, const double *a, const double *b, double *x, double tolerance,
+                    size_t restart, size_t max_iterations, int complex_, size_t *iterations,
+                    double *relative_residual, int *converged) {
+  dense_ctx d = {(const gz *)a, n};
+  size_t nn = n;
+  const gz_outcome o = gmres_core(dense_apply, &d, identity_op, &nn, (const gz *)b, (gz *)x, n,
+                                  tolerance, restart, max_iterations, complex_);
+  *iterations = o.iterations;
+  *relative_residual = o.relative_residual;
+  *converged = o.converged;
+  return FHO_OK;
+}
+
+/* ======================================================================== */
+/* iterative.rs: matrix-free port impedance by GMRES on the pFFT operator   */
+/* ======================================================================== */
+
+#define EXACT_DIAGONAL_MAX_LOOP_LENGTH 32 /* iterative.rs:94 */
+
+struct fho_iter {
+  size_t counts[6];
+  size_t n, m, p, loops;
+  fho_filament *filaments;
+  double *res;
+  int64_t *row_ptr, *branch;
+  double *sign;
+  fho_pfft *op;
+  double tolerance;
+  size_t restart, max_iterations;
+  double *r_ep, *l_ep, *r_pp, *l_pp; /* m x p, p x p row-major */
+  double *r_diag, *l_diag;
+  size_t last_iterations[64];
+};
+
+void fho_iter_destroy(fho_iter *s) {
+  if (!s) return;
+  free(s->filaments);
+  free(s->res);
+  free(s->row_ptr);
+  free(s->branch);
+  free(s->sign);
+  fho_pfft_destroy(s->op);
+  free(s->r_ep);
+  free(s->l_ep);
+  free(s->r_pp);
+  free(s->l_pp);
+  free(s->r_diag);
+  free(s->l_diag);
+  free(s);
+}
+
+/* iterative.rs:489-510: M_{0..loops} v */
+static void gather_loops(const fho_iter *s, size_t loops, const double *v, double *out) {
+  for (size_t i = 0; i < loops; ++i) {
+    double sum = 0.0;
+    for (int64_t e = s->row_ptr[i]; e < s->row_ptr[i + 1]; ++e) sum += v[s->branch[e]] * s->sign[e];
+    out[i] = sum;
+  }
+}
+
+/* iterative.rs:527-536 */
+static int loop_inductance(const fho_iter *s, size_t row, double *out) {
+  const int64_t b0 = s->row_ptr[row], b1 = s->row_ptr[row + 1];
+  double own = 0.0;
+  for (int64_t e = b0; e < b1; ++e) own += fho_self_inductance(&s->filaments[s->branch[e]]);
+  if (b1 - b0 > EXACT_DIAGONAL_MAX_LOOP_LENGTH) {
+    *out = own;
+    return FHO_OK;
+  }
+  double mutual = 0.0;
+  for (int64_t k = b0; k < b1; ++k)
+    for (int64_t l = k + 1; l < b1; ++l) {
+      double m;
+      int meth, res;
+      if (fho_mutual(&s->filaments[s->branch[k]], &s->filaments[s->branch[l]], &m, &meth, &res))
+        return FHO_E_KERNEL;
+      mutual += s->sign[k] * s->sign[l] * m;
+    }
+  *out = own + 2.0 * mutual;
+  return FHO_OK;
+}
+
+/* iterative.rs:155-246 */
+int fho_iter_create(const double *nodes, size_t n_nodes, const fho_segment *segments,
+                    size_t n_segments, const fho_grid *grids, const fho_port *ports,
+                    size_t n_ports, const fho_pfft_params *pp, double tolerance, size_t restart,
+                    size_t max_iterations, fho_iter **out) {
+  *out = NULL;
+  size_t n;
+  int st = fho_discretize_geometry(nodes, n_nodes, segments, n_segments, grids, NULL, 0, &n);
+  if (st) return st;
+  fho_iter *s = calloc(1, sizeof(fho_iter));
+  s->filaments = malloc(sizeof(fho_filament) * (n ? n : 1));
+  st = fho_discretize_geometry(nodes, n_nodes, segments, n_segments, grids, s->filaments, n, &n);
+  if (st) goto fail_;
+  size_t *per = malloc(sizeof(size_t) * (n_segments ? n_segments : 1));
+  for (size_t q = 0; q < n_segments; ++q) per[q] = (size_t)grids[q].nw * grids[q].nh;
+  size_t loops, internal, nnz;
+  st = fho_mesh_build(n_nodes, segments, n_segments, per, ports, n_ports, NULL, NULL, NULL,
+                      &loops, &internal, &nnz);
+  if (st) {
+    free(per);
+    goto fail_;
+  }
+  s->row_ptr = malloc(sizeof(int64_t) * (loops + 1));
+  s->branch = malloc(sizeof(int64_t) * (nnz ? nnz : 1));
+  s->sign = malloc(sizeof(double) * (nnz ? nnz : 1));
+  fho_mesh_build(n_nodes, segments, n_segments, per, ports, n_ports, s->row_ptr, s->branch,
+                 s->sign, &loops, &internal, &nnz);
+  free(per);
+  s->n = n;
+  s->m = internal;
+  s->p = loops - internal;
+  s->loops = loops;
+  s->tolerance = tolerance;
+  s->restart = restart;
+  s->max_iterations = max_iterations;
+  s->res = malloc(sizeof(double) * (n ? n : 1));
+  for (size_t i = 0; i < n; ++i) {
+    const fho_filament *f = &s->filaments[i];
+    s->res[i] = f->length / (f->sigma * area_of(f));
+  }
+  st = fho_pfft_create(s->filaments, n, pp, &s->op);
+  if (st) goto fail_;
+  const size_t m = s->m, p = s->p;
+  s->r_ep = calloc(m * p + 1, sizeof(double));
+  s->l_ep = calloc(m * p + 1, sizeof(double));
+  s->r_pp = calloc(p * p + 1, sizeof(double));
+  s->l_pp = calloc(p * p + 1, sizeof(double));
+  double *cur = malloc(sizeof(double) * (n ? n : 1)), *flux = malloc(sizeof(double) * (n ? n : 1));
+  double *drop = malloc(sizeof(double) * (n ? n : 1));
+  double *rc = malloc(sizeof(double) * loops), *lc = malloc(sizeof(double) * loops);
+  for (size_t q = 0; q < p; ++q) {
+    for (size_t b = 0; b < n; ++b) cur[b] = 0.0;
+    for (int64_t e = s->row_ptr[m + q]; e < s->row_ptr[m + q + 1]; ++e) cur[s->branch[e]] += s->sign[e];
+    fho_pfft_apply(s->op, cur, flux);
+    for (size_t b = 0; b < n; ++b) drop[b] = cur[b] * s->res[b];
+    gather_loops(s, loops, drop, rc);
+    gather_loops(s, loops, flux, lc);
+    for (size_t i = 0; i < m; ++i) {
+      s->r_ep[i * p + q] = rc[i];
+      s->l_ep[i * p + q] = lc[i];
+    }
+    for (size_t k = 0; k < p; ++k) {
+      s->r_pp[k * p + q] = rc[m + k];
+      s->l_pp[k * p + q] = lc[m + k];
+    }
+  }
+  free(cur);
+  free(flux);
+  free(drop);
+  free(rc);
+  free(lc);
+  /* (X + X^T) * 0.5 */
+  double *t = malloc(sizeof(double) * (p * p + 1));
+  for (int pass = 0; pass < 2; ++pass) {
+    double *X = pass ? s->l_pp : s->r_pp;
+    for (size_t a = 0; a < p; ++a)
+      for (size_t b = 0; b < p; ++b) t[a * p + b] = (X[a * p + b] + X[b * p + a]) * 0.5;
+    memcpy(X, t, sizeof(double) * p * p);
+  }
+  free(t);
+  s->r_diag = malloc(sizeof(double) * (m ? m : 1));
+  s->l_diag = malloc(sizeof(double) * (m ? m : 1));
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    double r = 0.0;
+    for (int64_t e = s->row_ptr[i]; e < s->row_ptr[i + 1]; ++e) r += s->res[s->branch[e]];
+    s->r_diag[i] = r;
+    if (loop_inductance(s, (size_t)i, &s->l_diag[i])) {
+#pragma omp atomic write
+      bad = 1;
+    }
+  }
+  if (bad) {
+    st = fail(FHO_E_KERNEL, 2, 0, 0, 0.0);
+    goto fail_;
+  }
+  s->counts[0] = n_nodes;
+  s->counts[1] = n_segments;
+  s->counts[2] = n;
+  s->counts[3] = loops;
+  s->counts[4] = internal;
+  s->counts[5] = loops - internal;
+  *out = s;
+  return FHO_OK;
+fail_:
+  fho_iter_destroy(s);
+  return st;
+}
+
+typedef struct {
+  const fho_iter *s;
+  double omega;
+  int dc;
+  double *re, *im, *fre, *fim, *drop_re, *drop_im; /* branch scratch */
+} ee_ctx;
+
+/* iterative.rs:431-463 */
+static void apply_ee(void *ctx, const gz *x, gz *y) {
+  ee_ctx *c = ctx;
+  const fho_iter *s = c->s;
+  const size_t n = s->n, m = s->m;
+  for (size_t b = 0; b < n; ++b) c->re[b] = c->im[b] = 0.0;
+  for (size_t i = 0; i < m; ++i)
+    for (int64_t e = s->row_ptr[i]; e < s->row_ptr[i + 1]; ++e) {
+      c->re[s->branch[e]] += x[i].re * s->sign[e];
+      c->im[s->branch[e]] += x[i].im * s->sign[e];
+    }
+  if (c->dc) {
+    for (size_t b = 0; b < n; ++b) {
+      c->drop_re[b] = c->re[b] * s->res[b];
+      c->drop_im[b] = c->im[b] * s->res[b];
+    }
+  } else {
+    fho_pfft_apply(s->op, c->re, c->fre);
+    fho_pfft_apply(s->op, c->im, c->fim);
+    for (size_t b = 0; b < n; ++b) {
+      const double r = s->res[b];
+      c->drop_re[b] = r * c->re[b] - c->omega * c->fim[b];
+      c->drop_im[b] = r * c->im[b] + c->omega * c->fre[b];
+    }
+  }
+  for (size_t i = 0; i < m; ++i) {
+    gz sum = {0.0, 0.0};
+    for (int64_t e = s->row_ptr[i]; e < s->row_ptr[i + 1]; ++e) {
+      sum.re += c->drop_re[s->branch[e]] * s->sign[e];
+      sum.im += c->drop_im[s->branch[e]] * s->sign[e];
+    }
+    y[i] = sum;
+  }
+}
+
+typedef struct {
+  const gz *d;
+  size_t m;
+} jacobi_ctx;
+static void jacobi(void *ctx, const gz *v, gz *out) {
+  const jacobi_ctx *j = ctx;
+  for (size_t i = 0; i < j->m; ++i) out[i] = gz_div(v[i], j->d[i]);
+}
+
+/* iterative.rs:298-326 (solve) + 467-486 (schur_complement_gmres); z is
+ * p x p complex interleaved.  iterations (p entries) may be NULL. */
+int fho_iter_impedance(fho_iter *s, double frequency, double *z, size_t *iterations) {
+  if (!(isfinite(frequency) && frequency >= 0.0))
+    return fail(FHO_E_INVALID_FREQUENCY, 0, 0, 0, frequency);
+  const size_t m = s->m, p = s->p, n = s->n;
+  const int dc = frequency == 0.0;
+  const double omega = 6.283185307179586 * frequency;
+  gz *zz = malloc(sizeof(gz) * (p * p + 1));
+  for (size_t a = 0; a < p; ++a)
+    for (size_t b = 0; b < p; ++b) {
+      zz[a * p + b].re = s->r_pp[a * p + b];
+      zz[a * p + b].im = dc ? 0.0 : omega * s->l_pp[a * p + b];
+    }
+  if (m > 0) {
+    gz *diag = malloc(sizeof(gz) * m);
+    for (size_t i = 0; i < m; ++i) {
+      diag[i].re = s->r_diag[i];
+      diag[i].im = dc ? 0.0 : omega * s->l_diag[i];
+    }
+    ee_ctx c = {s, omega, dc, NULL, NULL, NULL, NULL, NULL, NULL};
+    double *scratch = malloc(sizeof(double) * 6 * (n ? n : 1));
+    c.re = scratch;
+    c.im = scratch + n;
+    c.fre = scratch + 2 * n;
+    c.fim = scratch + 3 * n;
+    c.drop_re = scratch + 4 * n;
+    c.drop_im = scratch + 5 * n;
+    jacobi_ctx jc = {diag, m};
+    gz *rhs = malloc(sizeof(gz) * m), *x = malloc(sizeof(gz) * m);
+    gz *solved = malloc(sizeof(gz) * m * p);
+    for (size_t q = 0; q < p; ++q) {
+      for (size_t i = 0; i < m; ++i) {
+        rhs[i].re = s->r_ep[i * p + q];
+        rhs[i].im = dc ? 0.0 : omega * s->l_ep[i * p + q];
+        x[i].re = x[i].im = 0.0;
+      }
+      const gz_outcome o = gmres_core(apply_ee, &c, jacobi, &jc, rhs, x, m, s->tolerance,
+                                      s->restart, s->max_iterations, !dc);
+      if (iterations) iterations[q] = o.iterations;
+      if (!o.converged) {
+        free(diag);
+        free(scratch);
+        free(rhs);
+        free(x);
+        free(solved);
+        free(zz);
+        return fail(FHO_E_NOT_CONVERGED, 0, (int64_t)q, (int64_t)o.iterations, o.relative_residual);
+      }
+      for (size_t i = 0; i < m; ++i) solved[i * p + q] = x[i];
+    }
+    /* pp - ep^T solved */
+    for (size_t a = 0; a < p; ++a)
+      for (size_t b = 0; b < p; ++b) {
+        gz acc = {0.0, 0.0};
+        for (size_t i = 0; i < m; ++i) {
+          const gz e = {s->r_ep[i * p + a], dc ? 0.0 : omega * s->l_ep[i * p + a]};
+          acc = gz_add(acc, gz_mul(e, solved[i * p + b]));
+        }
+        zz[a * p + b] = gz_sub(zz[a * p + b], acc);
+      }
+    free(diag);
+    free(scratch);
+    free(rhs);
+    free(x);
+    free(solved);
+  }
+  for (size_t a = 0; a < p; ++a)
+    for (size_t b = 0; b < p; ++b) {
+      const gz e = gz_scale(gz_add(zz[a * p + b], zz[b * p + a]), 0.5);
+      z[2 * (a * p + b)] = e.re;
+      z[2 * (a * p + b) + 1] = dc ? 0.0 : e.im;
+    }
+  free(zz);
+  for (size_t q = 0; q < 2 * p * p; ++q)
+    if (!isfinite(z[q])) return fail(FHO_E_SINGULAR, 0, 0, 0, frequency);
+  return FHO_OK;
+}
+
+void fho_iter_counts(const fho_iter *s, size_t counts[6]) { memcpy(counts, s->counts, sizeof(s->counts)); }
+const fho_pfft *fho_iter_operator(const fho_iter *s) { return s->op; }
+const fho_filament *fho_iter_filaments(const fho_iter *s) { return s->filaments; }

@@ -31,6 +31,7 @@ enum {
   FHO_E_INVALID_GRADING = 6,
   FHO_E_INVALID_FREQUENCY = 7,
   FHO_E_SINGULAR = 8,
+  FHO_E_NOT_CONVERGED = 9,
   FHO_E_NO_MEMORY = 11
 };
 
@@ -162,6 +163,55 @@ int fho_system_sweep(const fho_system *s, const double *freqs, size_t K, double 
 int fho_solve(const double *nodes, size_t n_nodes, const fho_segment *segments,
               size_t n_segments, const fho_grid *grids, const fho_port *ports, size_t n_ports,
               const double *freqs, size_t K, double *z_out, int threads);
+
+
+/* ---- pfft/{mod,grid,near,fft}.rs: precorrected-FFT operator ---- */
+typedef struct {
+  int auto_grid;             /* 1: GridSpacing::Auto; 0: GridSpacing::Fixed */
+  double cells_per_filament; /* Auto */
+  double spacing;            /* Fixed, metres */
+  double near_field_radius;  /* in grid spacings */
+  int interpolation_order;
+} fho_pfft_params;
+typedef struct fho_pfft fho_pfft;
+/* mod.rs:332-445.  FHO_E_INVALID_ARGUMENT with code 1-4 for an invalid
+ * parameter, code 5 for a grid beyond 2^26 padded points; FHO_E_KERNEL for a
+ * near pair whose exact kernel fails. */
+int fho_pfft_create(const fho_filament *f, size_t n, const fho_pfft_params *params,
+                    fho_pfft **out);
+void fho_pfft_destroy(fho_pfft *op);
+/* mod.rs:482-528: y = L x */
+void fho_pfft_apply(const fho_pfft *op, const double *x, double *y);
+void fho_pfft_stats(const fho_pfft *op, double *spacing, size_t dims[3], size_t padded[3],
+                    size_t *projection_entries, size_t *near_entries);
+int fho_pfft_data(const fho_pfft *op, size_t *proj_off, uint32_t *proj_idx, double *proj_w,
+                  size_t *near_off, uint32_t *near_col, double *near_val, double origin[3]);
+/* fft.rs convolution of grid arrays (dims of the operator's grid). */
+int fho_pfft_convolve(const fho_pfft *op, const double *a, const double *b, double *pa,
+                      double *pb);
+double fho_auto_spacing(const double extent[3], double cells);
+size_t fho_smooth_size(size_t n);
+size_t fho_axis_stencil(const double origin[3], double h, const size_t dims[3], int m, int axis,
+                        double x, double *weights);
+
+/* ---- gmres.rs on a dense column-major matrix (interleaved complex) ---- */
+int fho_gmres_dense(size_t n, const double *a, const double *b, double *x, double tolerance,
+                    size_t restart, size_t max_iterations, int complex_, size_t *iterations,
+                    double *relative_residual, int *converged);
+
+/* ---- iterative.rs: IterativeSystem ---- */
+typedef struct fho_iter fho_iter;
+int fho_iter_create(const double *nodes, size_t n_nodes, const fho_segment *segments,
+                    size_t n_segments, const fho_grid *grids, const fho_port *ports,
+                    size_t n_ports, const fho_pfft_params *pfft, double tolerance, size_t restart,
+                    size_t max_iterations, fho_iter **out);
+void fho_iter_destroy(fho_iter *s);
+/* z: p x p complex interleaved; FHO_E_NOT_CONVERGED (index0 = port,
+ * index1 = iterations, value = relative residual) */
+int fho_iter_impedance(fho_iter *s, double frequency, double *z, size_t *iterations);
+void fho_iter_counts(const fho_iter *s, size_t counts[6]);
+const fho_pfft *fho_iter_operator(const fho_iter *s);
+const fho_filament *fho_iter_filaments(const fho_iter *s);
 
 #ifdef __cplusplus
 }
