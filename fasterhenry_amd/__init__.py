@@ -1,6 +1,6 @@
 """fasterhenry_amd -- MI355X-native (gfx950) PEEC extraction hot path.
 
-A drop-in for the dense solve path of fasterhenry 0.2.0: the same problem
+A drop-in for the solve paths of fasterhenry 0.2.0: the same problem
 types (Geometry, Port, Discretization, SolverChoice,
 DENSE_PATH_MAX_FILAMENTS) and entry points (solve, MeshSystem), computed by
 hand-written HIP kernels in libfasterhenry_gpu.so through its C ABI
@@ -10,6 +10,6 @@ solve loads the native library and raises if it or the device is missing.
 from .errors import *  # noqa: F401,F403
 from .model import *  # noqa: F401,F403
 from .model import DENSE_PATH_MAX_FILAMENTS, MU0  # noqa: F401
-from .api import (Counts, MeshSystem, SweepResult, Timing, device_math,  # noqa: F401
-                  discretize_geometry, discretize_packed, partial_inductance_matrix, solve,
-                  version)
+from .api import (Counts, IterativeSystem, MeshSystem, PfftOperator, SweepResult,  # noqa: F401
+                  Timing, device_math, discretize_geometry, discretize_packed, gmres_dense,
+                  iterative_params, partial_inductance_matrix, pfft_params, solve, version)

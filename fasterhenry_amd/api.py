@@ -6,6 +6,8 @@ Mirrors fasterhenry 0.2.0:
 * ``SweepResult`` (schema 1) -- result.rs:47-94
 * ``partial_inductance_matrix`` -- inductance/batch.rs:246-253
 * ``discretize_geometry`` -- solve.rs:947-975
+* ``IterativeSystem`` -- iterative.rs:155-536; ``PfftOperator`` --
+  pfft/mod.rs:270-528; ``gmres_dense`` -- gmres.rs:91-206
 
 Every call runs on the GPU through libfasterhenry_gpu.so; there is no CPU
 fallback.  ``precision`` selects the factorization arithmetic: "fp64"
@@ -197,16 +199,190 @@ class MeshSystem:
         return dict(zip(["r_ee", "l_ee", "r_ep", "l_ep", "r_pp", "l_pp"], out))
 
 
+def pfft_params(spacing: Optional[float] = None, cells_per_filament: float = 8.0,
+                near_field_radius: float = 8.0, interpolation_order: int = 5) -> native.PfftParams:
+    """PfftParams (pfft/mod.rs:183-231): GridSpacing::Fixed(spacing) when
+    spacing is given, else GridSpacing::Auto{cells_per_filament}."""
+    p = native.PfftParams()
+    native.lib().fh_pfft_default_params(C.byref(p))
+    p.auto_grid = 0 if spacing is not None else 1
+    p.spacing = float(spacing or 0.0)
+    p.cells_per_filament = float(cells_per_filament)
+    p.near_field_radius = float(near_field_radius)
+    p.interpolation_order = int(interpolation_order)
+    return p
+
+
+def iterative_params(pfft: Optional[native.PfftParams] = None, tolerance: float = 1e-10,
+                     restart: int = 100, max_iterations: int = 5000) -> native.IterativeParams:
+    """IterativeParams (iterative.rs:98-105) with GmresParams (gmres.rs:29-60)."""
+    p = native.IterativeParams()
+    native.lib().fh_iterative_default_params(C.byref(p))
+    if pfft is not None:
+        p.pfft = pfft
+    p.gmres.tolerance = float(tolerance)
+    p.gmres.restart = int(restart)
+    p.gmres.max_iterations = int(max_iterations)
+    return p
+
+
+class IterativeSystem:
+    """The matrix-free mesh system (iterative.rs:155-536): pFFT operator,
+    port columns and Jacobi diagonals resident on the GPU; one GMRES solve
+    per port and frequency."""
+
+    def __init__(self, handle, packed: PackedProblem, ports: Sequence[Port]):
+        self._h = handle
+        self._packed = packed
+        self.ports = list(ports)
+        self._counts = _counts(handle)
+
+    @classmethod
+    def assemble(cls, geometry: Geometry, ports: Sequence[Port], discretization: Discretization,
+                 params: Optional[native.IterativeParams] = None) -> "IterativeSystem":
+        return cls.assemble_packed(pack(geometry, ports, discretization), ports, params)
+
+    @classmethod
+    def assemble_packed(cls, packed: PackedProblem, ports: Sequence[Port],
+                        params: Optional[native.IterativeParams] = None) -> "IterativeSystem":
+        L = native.lib()
+        h = C.c_void_p()
+        prob = native.problem_struct(packed)
+        params = params if params is not None else iterative_params()
+        native.check(L.fh_system_create_iterative(C.byref(prob), C.byref(params), C.byref(h)))
+        return cls(h, packed, ports)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            native.lib().fh_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, enable: bool) -> None:
+        native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
+
+    def counts(self) -> Counts:
+        return self._counts
+
+    def operator_stats(self) -> dict:
+        """IterativeSystem::operator_stats (iterative.rs:255-258)."""
+        st = native.PfftStats()
+        native.check(native.lib().fh_system_operator_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def filaments(self) -> np.ndarray:
+        out = np.zeros(self._counts.filaments, dtype=FILAMENT_DTYPE)
+        native.check(native.lib().fh_system_filaments(self._h, native.ptr(out)))
+        return out
+
+    sweep_raw = MeshSystem.sweep_raw
+
+    def sweep(self, frequencies) -> SweepResult:
+        """Z at every frequency (iterative.rs:387-404)."""
+        z, info = self.sweep_raw(frequencies)
+        timing = Timing("gfx950", info.inductance_s, info.assembly_s, info.solve_s,
+                        info.assembly_s + info.solve_s)
+        return SweepResult(self.ports, [float(x) for x in np.asarray(frequencies, dtype=float)], z,
+                           self._counts, timing,
+                           extra={"gmres_iterations_max": info.gmres_iterations_max,
+                                  "gmres_iterations": info.gmres_iterations,
+                                  "gmres_residual_max": info.gmres_residual_max})
+
+    def impedance(self, frequency: float) -> np.ndarray:
+        """Z at one frequency (iterative.rs:270-272)."""
+        return self.sweep_raw([frequency])[0][0]
+
+
+class PfftOperator:
+    """The precorrected-FFT partial-inductance operator (pfft/mod.rs:270-528)
+    of a FILAMENT_DTYPE array, resident on the GPU."""
+
+    def __init__(self, filaments: np.ndarray, params: Optional[native.PfftParams] = None):
+        self._f = np.ascontiguousarray(filaments, dtype=FILAMENT_DTYPE)
+        self.n = len(self._f)
+        self.params = params if params is not None else pfft_params()
+        self._h = C.c_void_p()
+        native.check(native.lib().fh_pfft_create(native.ptr(self._f) if self.n else None, self.n,
+                                                 C.byref(self.params), C.byref(self._h)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            native.lib().fh_pfft_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, x) -> np.ndarray:
+        """y = L x (pfft/mod.rs:482-528) for one vector (n,) or a batch (V, n)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        nvec = 1 if x.ndim == 1 else x.shape[0]
+        if x.shape[-1] != self.n:
+            raise InvalidArgument(f"vector length {x.shape[-1]} != {self.n} filaments")
+        native.check(native.lib().fh_pfft_apply(self._h, native.ptr(x), native.ptr(y), nvec))
+        return y
+
+    def stats(self) -> dict:
+        """PfftStats (pfft/mod.rs:255-270)."""
+        st = native.PfftStats()
+        native.check(native.lib().fh_pfft_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def data(self) -> dict:
+        """The projection and near-field CSR arrays, copied from the device."""
+        st = self.stats()
+        pe, ne = st["projection_entries"], st["near_entries"]
+        proj_off = np.zeros(self.n + 1, dtype=np.int64)
+        proj_idx = np.zeros(max(pe, 1), dtype=np.uint32)
+        proj_w = np.zeros(max(pe, 1))
+        near_off = np.zeros(self.n + 1, dtype=np.int64)
+        near_col = np.zeros(max(ne, 1), dtype=np.uint32)
+        near_val = np.zeros(max(ne, 1))
+        native.check(native.lib().fh_pfft_data(self._h, *(native.ptr(a) for a in (
+            proj_off, proj_idx, proj_w, near_off, near_col, near_val))))
+        return dict(proj_off=proj_off, proj_idx=proj_idx[:pe], proj_w=proj_w[:pe],
+                    near_off=near_off, near_col=near_col[:ne], near_val=near_val[:ne], **st)
+
+
+def gmres_dense(a: np.ndarray, b: np.ndarray, x0=None, tolerance: float = 1e-10,
+                restart: int = 100, max_iterations: int = 5000,
+                complex_arith: bool = True) -> tuple:
+    """gmres (gmres.rs:91-206) on the GPU for a dense matrix, no
+    preconditioner.  Returns (x, iterations, relative_residual, converged)."""
+    a = np.asarray(a, dtype=np.complex128)
+    n = a.shape[0]
+    af = np.asfortranarray(a)
+    bb = np.ascontiguousarray(b, dtype=np.complex128)
+    x = np.zeros(n, dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+    prm = native.GmresParams(float(tolerance), int(restart), int(max_iterations))
+    it, rr, cv = C.c_uint64(), C.c_double(), C.c_int32()
+    native.check(native.lib().fh_gmres_dense(n, af.ctypes.data_as(C.c_void_p), native.ptr(bb),
+                                             native.ptr(x), C.byref(prm),
+                                             1 if complex_arith else 0, C.byref(it),
+                                             C.byref(rr), C.byref(cv)))
+    return x, it.value, rr.value, bool(cv.value)
+
+
 def solve(geometry: Geometry, ports: Sequence[Port], discretization: Discretization,
           frequencies, solver: SolverChoice = SolverChoice.AUTO,
           precision: str = "fp64") -> SweepResult:
     """Assemble and sweep (solve.rs:931-938).  SolverChoice.AUTO resolves
-    against DENSE_PATH_MAX_FILAMENTS; this build provides the dense path."""
+    against DENSE_PATH_MAX_FILAMENTS: the dense GPU LU up to it, the pFFT +
+    GMRES path (default IterativeParams) above."""
     n = discretization.filament_count(geometry)
-    if solver.resolve(n) is not Solver.DENSE:
-        raise SolveError(f"{n} filaments resolve to the iterative path, which this build does "
-                         "not provide; pass solver=SolverChoice.DENSE", status=10)
-    system = MeshSystem.assemble(geometry, ports, discretization, precision, SolverChoice.DENSE)
+    if solver.resolve(n) is Solver.DENSE:
+        system = MeshSystem.assemble(geometry, ports, discretization, precision, SolverChoice.DENSE)
+    else:
+        system = IterativeSystem.assemble(geometry, ports, discretization)
     try:
         return system.sweep(frequencies)
     finally:

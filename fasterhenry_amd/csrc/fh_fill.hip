@@ -822,6 +822,90 @@ __global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__res
   }
 }
 
+// One queued pair on one wavefront (the deferred methods): value
+// 1e-7 * integral, its Method, and false for the divergent sampled case.
+__device__ void deferred_eval(const fh_filament &fa, const fh_filament &fb, uint32_t kind,
+                              double *lds, Frame *stack, int lane, double &v, int8_t &method,
+                              bool &ok) {
+  PairPlan p;
+  plan_pair(fa, fb, p);
+  double integral = 0.0;
+  method = kPointQuadrature;
+  ok = true;
+  if (kind == kHeavyPoint) {
+    // neumann.rs:237-266, lanes over the outer cloud.
+    const int na = p.oa[0] * p.oa[1] * p.oa[2];
+    __syncthreads();
+    for (int idx = lane; idx < na; idx += 64) {
+      D3 pa;
+      double wa;
+      cloud_point(p.a, p.oa, idx, pa, wa);
+      lds[idx] = wa * inner_sum(pa, p.b, p.ob);
+    }
+    __syncthreads();
+    double total = 0.0;
+    for (int idx = 0; idx < na; ++idx) total += lds[idx];
+    integral = total * p.cos;
+  } else if (kind == kAligned) {
+    const double in = aligned_dispatch_wave(p.ab, method, stack, lds);
+    const double scale = signum(p.cos) / (p.area_a * p.area_b);
+    integral = in * scale;
+  } else {
+    // neumann.rs:280-325, lanes over the sample pairs.
+    method = kSampledFilaments;
+    int oa[2], ob[2];
+    const double halves[4] = {p.a.half[1], p.a.half[2], p.b.half[1], p.b.half[2]};
+    int *targets[4] = {&oa[0], &oa[1], &ob[0], &ob[1]};
+    for (int t = 0; t < 4; ++t) {
+      const int o = order_for(c_sample_th, kSampleMaxOrder, halves[t], p.gap);
+      *targets[t] = o ? o : kSampleMaxOrder + (t >= 2 ? 1 : 0);
+    }
+    const int na = oa[0] * oa[1], nb = ob[0] * ob[1], total_pairs = na * nb;
+    const double la = 2.0 * p.a.half[0], lb = 2.0 * p.b.half[0];
+    double value = 0.0;
+    int bad = 0;
+    for (int base = 0; base < total_pairs; base += 64) {
+      const int idx = base + lane;
+      double term = 0.0;
+      if (idx < total_pairs) {
+        const int ia = idx / nb, ib = idx % nb;
+        double xw, ww, xh, wh;
+        rule_on(oa[0], ia / oa[1], -0.5, 0.5, xw, ww);
+        rule_on(oa[1], ia % oa[1], -0.5, 0.5, xh, wh);
+        const D3 pa = p.a.centre + (p.a.axes[1] * (2.0 * p.a.half[1] * xw) +
+                                    p.a.axes[2] * (2.0 * p.a.half[2] * xh));
+        const double wa = ww * wh;
+        rule_on(ob[0], ib / ob[1], -0.5, 0.5, xw, ww);
+        rule_on(ob[1], ib % ob[1], -0.5, 0.5, xh, wh);
+        const D3 pb = p.b.centre + (p.b.axes[1] * (2.0 * p.b.half[1] * xw) +
+                                    p.b.axes[2] * (2.0 * p.b.half[2] * xh));
+        const double wb = ww * wh;
+        double kernel = 0.0;
+        bool good;
+        if (p.parallel) {
+          const D3 axis = p.a.axes[0];
+          const D3 delta = pb - pa;
+          const double along = dot(delta, axis);
+          const double rho = norm(delta - axis * along);
+          good = lines_parallel(la, lb, along + 0.5 * (la - lb), rho, kernel);
+        } else {
+          const D3 ha = p.a.axes[0] * p.a.half[0], hb = p.b.axes[0] * p.b.half[0];
+          good = lines_skew(pa - ha, pa + ha, pb - hb, pb + hb, 0.0, kernel);
+        }
+        if (!good) bad = 1;
+        term = wa * wb * kernel;
+      }
+      const int cnt = total_pairs - base < 64 ? total_pairs - base : 64;
+      value = wave_ordered_sum(value, term, cnt, lds);
+    }
+    bad = __syncthreads_or(bad);
+    if (bad) ok = false;
+    const double direction = p.parallel ? signum(p.cos) : p.cos;
+    integral = value * direction;
+  }
+  v = 1.0e-7 * integral;
+}
+
 __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__restrict__ F, int n,
                                                            double *__restrict__ L,
                                                            int8_t *__restrict__ methods,
@@ -836,83 +920,10 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
     const uint2 q = queue[e];
     const uint32_t kind = q.x >> 30;
     const int i = static_cast<int>(q.x & 0x3fffffffu), j = static_cast<int>(q.y);
-    PairPlan p;
-    plan_pair(F[i], F[j], p);
-    double integral = 0.0;
-    int8_t method = kPointQuadrature;
-    bool ok = true;
-    if (kind == kHeavyPoint) {
-      // neumann.rs:237-266, lanes over the outer cloud.
-      const int na = p.oa[0] * p.oa[1] * p.oa[2];
-      __syncthreads();
-      for (int idx = lane; idx < na; idx += 64) {
-        D3 pa;
-        double wa;
-        cloud_point(p.a, p.oa, idx, pa, wa);
-        lds[idx] = wa * inner_sum(pa, p.b, p.ob);
-      }
-      __syncthreads();
-      double total = 0.0;
-      for (int idx = 0; idx < na; ++idx) total += lds[idx];
-      integral = total * p.cos;
-    } else if (kind == kAligned) {
-      const double in = aligned_dispatch_wave(p.ab, method, stack, lds);
-      const double scale = signum(p.cos) / (p.area_a * p.area_b);
-      integral = in * scale;
-    } else {
-      // neumann.rs:280-325, lanes over the sample pairs.
-      method = kSampledFilaments;
-      int oa[2], ob[2];
-      const double halves[4] = {p.a.half[1], p.a.half[2], p.b.half[1], p.b.half[2]};
-      int *targets[4] = {&oa[0], &oa[1], &ob[0], &ob[1]};
-      for (int t = 0; t < 4; ++t) {
-        const int o = order_for(c_sample_th, kSampleMaxOrder, halves[t], p.gap);
-        *targets[t] = o ? o : kSampleMaxOrder + (t >= 2 ? 1 : 0);
-      }
-      const int na = oa[0] * oa[1], nb = ob[0] * ob[1], total_pairs = na * nb;
-      const double la = 2.0 * p.a.half[0], lb = 2.0 * p.b.half[0];
-      double value = 0.0;
-      int bad = 0;
-      for (int base = 0; base < total_pairs; base += 64) {
-        const int idx = base + lane;
-        double term = 0.0;
-        if (idx < total_pairs) {
-          const int ia = idx / nb, ib = idx % nb;
-          double xw, ww, xh, wh;
-          rule_on(oa[0], ia / oa[1], -0.5, 0.5, xw, ww);
-          rule_on(oa[1], ia % oa[1], -0.5, 0.5, xh, wh);
-          const D3 pa = p.a.centre + (p.a.axes[1] * (2.0 * p.a.half[1] * xw) +
-                                      p.a.axes[2] * (2.0 * p.a.half[2] * xh));
-          const double wa = ww * wh;
-          rule_on(ob[0], ib / ob[1], -0.5, 0.5, xw, ww);
-          rule_on(ob[1], ib % ob[1], -0.5, 0.5, xh, wh);
-          const D3 pb = p.b.centre + (p.b.axes[1] * (2.0 * p.b.half[1] * xw) +
-                                      p.b.axes[2] * (2.0 * p.b.half[2] * xh));
-          const double wb = ww * wh;
-          double kernel = 0.0;
-          bool good;
-          if (p.parallel) {
-            const D3 axis = p.a.axes[0];
-            const D3 delta = pb - pa;
-            const double along = dot(delta, axis);
-            const double rho = norm(delta - axis * along);
-            good = lines_parallel(la, lb, along + 0.5 * (la - lb), rho, kernel);
-          } else {
-            const D3 ha = p.a.axes[0] * p.a.half[0], hb = p.b.axes[0] * p.b.half[0];
-            good = lines_skew(pa - ha, pa + ha, pb - hb, pb + hb, 0.0, kernel);
-          }
-          if (!good) bad = 1;
-          term = wa * wb * kernel;
-        }
-        const int cnt = total_pairs - base < 64 ? total_pairs - base : 64;
-        value = wave_ordered_sum(value, term, cnt, lds);
-      }
-      bad = __syncthreads_or(bad);
-      if (bad) ok = false;
-      const double direction = p.parallel ? signum(p.cos) : p.cos;
-      integral = value * direction;
-    }
-    const double v = 1.0e-7 * integral;
+    double v;
+    int8_t method;
+    bool ok;
+    deferred_eval(F[i], F[j], kind, lds, stack, lane, v, method, ok);
     if (lane == 0) {
       if (!ok) {
         record_error(err, i, j, n, 1);
@@ -924,6 +935,62 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
           methods[static_cast<size_t>(i) * n + j] = method;
           methods[static_cast<size_t>(j) * n + i] = method;
         }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Pair list: out[k] = partial inductance of filaments pairs[k] = (i, j).
+// Cheap pairs are finished by one thread each; the rest are queued (pair
+// index | kind << 30) for one wavefront each, as in the tile fill.
+__global__ void __launch_bounds__(256) pairs_kernel(const fh_filament *__restrict__ F, int n,
+                                                    const uint2 *__restrict__ pairs, size_t count,
+                                                    double *__restrict__ out,
+                                                    uint2 *__restrict__ queue,
+                                                    unsigned int *__restrict__ queue_count,
+                                                    unsigned long long *__restrict__ err) {
+  const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (k >= count) return;
+  const int i = static_cast<int>(pairs[k].x), j = static_cast<int>(pairs[k].y);
+  PairPlan p;
+  plan_pair(F[i], F[j], p);
+  if (p.orthogonal) {
+    out[k] = 0.0;
+  } else if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
+    const double value = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
+    out[k] = value;
+    if (!__builtin_isfinite(value)) record_error(err, i, j, n, 2);
+  } else {
+    const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
+    const unsigned int slot = atomicAdd(queue_count, 1u);
+    queue[slot] = make_uint2(static_cast<uint32_t>(k) | (kind << 30), 0u);
+  }
+}
+
+__global__ void __launch_bounds__(64) pairs_deferred_kernel(
+    const fh_filament *__restrict__ F, int n, const uint2 *__restrict__ pairs,
+    double *__restrict__ out, const uint2 *__restrict__ queue,
+    const unsigned int *__restrict__ queue_count, unsigned long long *__restrict__ err) {
+  __shared__ double lds[kCloudCapacity];
+  __shared__ Frame stack[kStackCapacity];
+  const unsigned int count = *queue_count;
+  const int lane = static_cast<int>(threadIdx.x);
+  for (unsigned int e = blockIdx.x; e < count; e += gridDim.x) {
+    const uint2 q = queue[e];
+    const uint32_t kind = q.x >> 30;
+    const size_t k = q.x & 0x3fffffffu;
+    const int i = static_cast<int>(pairs[k].x), j = static_cast<int>(pairs[k].y);
+    double v;
+    int8_t method;
+    bool ok;
+    deferred_eval(F[i], F[j], kind, lds, stack, lane, v, method, ok);
+    if (lane == 0) {
+      if (!ok) {
+        record_error(err, i, j, n, 1);
+      } else {
+        if (!__builtin_isfinite(v)) record_error(err, i, j, n, 2);
+        out[k] = v;
       }
     }
     __syncthreads();
@@ -986,6 +1053,47 @@ fh_status fill_inductance(const fh_filament *d_F, size_t n, double *d_L, int8_t 
   }
   return FH_OK;
 }
+
+fh_status fill_pairs(const fh_filament *d_F, size_t n, const uint2 *d_pairs, size_t count,
+                     double *d_out, hipStream_t stream) {
+  if (count == 0) return FH_OK;
+  if (n >= (1u << 30) || count >= (1u << 30))
+    return set_error(FH_E_INVALID_ARGUMENT, "too many filaments or pairs");
+  fh_status st = upload_tables();
+  if (st) return st;
+  DeviceArray<uint2> queue;
+  DeviceArray<unsigned int> qcount;
+  DeviceArray<unsigned long long> err;
+  if ((st = queue.allocate(count))) return st;
+  if ((st = qcount.allocate(1))) return st;
+  if ((st = err.allocate(1))) return st;
+  FH_HIP_TRY(hipMemsetAsync(qcount.ptr, 0, sizeof(unsigned int), stream));
+  FH_HIP_TRY(hipMemsetAsync(err.ptr, 0xff, sizeof(unsigned long long), stream));
+  pairs_kernel<<<static_cast<unsigned>((count + 255) / 256), 256, 0, stream>>>(
+      d_F, static_cast<int>(n), d_pairs, count, d_out, queue.ptr, qcount.ptr, err.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  int dev = 0, cus = 256;
+  FH_HIP_TRY(hipGetDevice(&dev));
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  pairs_deferred_kernel<<<static_cast<unsigned>(cus) * 16, 64, 0, stream>>>(
+      d_F, static_cast<int>(n), d_pairs, d_out, queue.ptr, qcount.ptr, err.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  unsigned long long key = 0;
+  FH_HIP_TRY(hipMemcpyAsync(&key, err.ptr, sizeof(key), hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));
+  if (key != ~0ULL) {
+    const int code = static_cast<int>(key & 3u);
+    const unsigned long long pair = key >> 2;
+    const int64_t i = static_cast<int64_t>(pair / n), j = static_cast<int64_t>(pair % n);
+    return set_error(FH_E_KERNEL,
+                     "partial inductance of filaments " + std::to_string(i) + " and " +
+                         std::to_string(j) + (code == 1 ? " is not resolved" : " is not finite"),
+                     i, j, 0.0, code);
+  }
+  return FH_OK;
+}
+
+void gauss_rule_host(int n, double *nodes, double *weights) { host_rule(n, nodes, weights); }
 
 fh_status device_math(const double *x, size_t n, double *log_out, double *atan_out) {
   if (n == 0) return FH_OK;

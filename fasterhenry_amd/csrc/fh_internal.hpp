@@ -96,5 +96,11 @@ void congruence_resistance(const LoopBasis &basis, const std::vector<double> &r,
 
 // Device-side portable math check (fh_fill.hip).
 fh_status device_math(const double *x, size_t n, double *log_out, double *atan_out);
+// Partial inductances of the filament pairs d_pairs[k] = (i, j) into d_out[k]
+// (device arrays), bit-identical to the same entries of fill_inductance.
+fh_status fill_pairs(const fh_filament *d_filaments, size_t n, const uint2 *d_pairs, size_t count,
+                     double *d_out, hipStream_t stream);
+// The n-point Gauss-Legendre rule on [-1, 1] (gauss.rs:25-70), host libm.
+void gauss_rule_host(int n, double *nodes, double *weights);
 
 }  // namespace fh

@@ -14,7 +14,9 @@
 #include <memory>
 
 #include "fh_internal.hpp"
+#include "fh_iterative.hpp"
 #include "fh_ldlt.hpp"
+#include "fh_pfft.hpp"
 
 using Clock = std::chrono::steady_clock;
 
@@ -44,6 +46,8 @@ using namespace fh;
 
 struct fh_system {
   hipStream_t stream = nullptr;
+  fh_solver solver = FH_SOLVER_DENSE;
+  std::unique_ptr<IterativeSystem> iter;
   fh_precision precision = FH_FP64_PLAIN;
   bool profiling = false;
   size_t counts[6] = {0, 0, 0, 0, 0, 0};
@@ -64,10 +68,22 @@ struct fh_system {
   LdltWorkspace ws;
   double inductance_s = 0.0, assembly_s = 0.0;
   ~fh_system() {
+    iter.reset();
     d_filaments.release();
     d_L.release();
     d_lmesh.release();
     ws.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct fh_pfft {
+  hipStream_t stream = nullptr;
+  PfftOperator op;
+  DeviceArray<double> dx, dy;
+  ~fh_pfft() {
+    dx.release();
+    dy.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -146,10 +162,15 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
   size_t nfil = 0;
   for (size_t s = 0; s < problem->n_segments && problem->grids; ++s)
     nfil += static_cast<size_t>(problem->grids[s].nw) * problem->grids[s].nh;
-  if (fh_resolve(choice, nfil) != FH_SOLVER_DENSE)
-    return set_error(FH_E_UNSUPPORTED,
-                     "the iterative (pFFT + GMRES) path is not provided by this build; "
-                     "choose FH_DENSE");
+  if (choice != FH_AUTO && choice != FH_DENSE && choice != FH_ITERATIVE)
+    return set_error(FH_E_INVALID_ARGUMENT, "unknown solver choice");
+  if (fh_resolve(choice, nfil) != FH_SOLVER_DENSE) {
+    fh_iterative_params params;
+    fh_iterative_default_params(&params);
+    fh_status st = fh_system_create_iterative(problem, &params, out);
+    if (st == FH_OK) (*out)->precision = precision;
+    return st;
+  }
   fh_status st = check_device();
   if (st) return st;
   std::unique_ptr<fh_system> s(new fh_system());
@@ -245,6 +266,68 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
   return FH_OK;
 }
 
+void fh_iterative_default_params(fh_iterative_params *out) {
+  if (!out) return;
+  fh_pfft_default_params(&out->pfft);
+  out->gmres.tolerance = 1e-10;      // gmres.rs:45
+  out->gmres.restart = 100;          // gmres.rs:47
+  out->gmres.max_iterations = 5000;  // gmres.rs:49
+}
+
+void fh_pfft_default_params(fh_pfft_params *out) {
+  if (!out) return;
+  out->auto_grid = 1;
+  out->interpolation_order = 5;   // pfft/mod.rs:217
+  out->cells_per_filament = 8.0;  // pfft/mod.rs:213
+  out->spacing = 0.0;
+  out->near_field_radius = 8.0;   // pfft/mod.rs:215
+}
+
+fh_status fh_system_create_iterative(const fh_problem *problem, const fh_iterative_params *params,
+                                     fh_system **out) {
+  clear_error();
+  if (!problem || !params || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  const auto t0 = Clock::now();
+  fh_status st = check_device();
+  if (st) return st;
+  std::unique_ptr<fh_system> s(new fh_system());
+  s->solver = FH_SOLVER_ITERATIVE;
+  if ((st = discretize_problem(*problem, s->filaments))) return st;
+  const size_t n = s->filaments.size();
+  std::vector<size_t> per(problem->n_segments);
+  for (size_t q = 0; q < problem->n_segments; ++q)
+    per[q] = static_cast<size_t>(problem->grids[q].nw) * problem->grids[q].nh;
+  if ((st = build_loop_basis(*problem, per, s->basis))) return st;
+  s->resistances.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const fh_filament &f = s->filaments[i];
+    s->resistances[i] = f.length / (f.sigma * (f.width * f.height));  // solve.rs:941-943
+  }
+  FH_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  s->iter.reset(new IterativeSystem());
+  if ((st = s->iter->build(s->filaments, s->resistances, s->basis, *params, s->stream))) return st;
+  s->inductance_s = s->iter->operator_seconds();
+  const size_t loops = s->basis.loops;
+  s->counts[0] = problem->n_nodes;
+  s->counts[1] = problem->n_segments;
+  s->counts[2] = n;
+  s->counts[3] = loops;
+  s->counts[4] = s->basis.internal;
+  s->counts[5] = loops - s->basis.internal;
+  s->assembly_s = seconds_since(t0);
+  *out = s.release();
+  return FH_OK;
+}
+
+fh_status fh_system_operator_stats(const fh_system *s, fh_pfft_stats *out) {
+  clear_error();
+  if (!s || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (!s->iter) return set_error(FH_E_UNSUPPORTED, "a dense system has no pFFT operator");
+  *out = s->iter->op().stats();
+  return FH_OK;
+}
+
 void fh_system_destroy(fh_system *system) { delete system; }
 
 fh_status fh_system_set_precision(fh_system *s, fh_precision precision) {
@@ -280,6 +363,7 @@ fh_status fh_system_filaments(const fh_system *s, fh_filament *out) {
 fh_status fh_system_inductance(const fh_system *s, double *L_out) {
   clear_error();
   if (!s || !L_out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (s->iter) return set_error(FH_E_UNSUPPORTED, "an iterative system holds no dense L");
   const size_t n = s->filaments.size();
   if (n) FH_HIP_TRY(hipMemcpy(L_out, s->d_L.ptr, n * n * sizeof(double), hipMemcpyDeviceToHost));
   return FH_OK;
@@ -289,6 +373,7 @@ fh_status fh_system_blocks(const fh_system *s, double *r_ee, double *l_ee, doubl
                            double *l_ep, double *r_pp, double *l_pp) {
   clear_error();
   if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  if (s->iter) return set_error(FH_E_UNSUPPORTED, "an iterative system holds no dense mesh blocks");
   const size_t loops = s->basis.loops, m = s->basis.internal, p = loops - m;
   std::vector<double> lm(loops * loops), rm(loops * loops, 0.0);
   if (loops)
@@ -324,6 +409,55 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
                        static_cast<int64_t>(k), 0, freqs[k]);
   const auto t0 = Clock::now();
   const size_t m = s->basis.internal, p = s->basis.loops - m;
+  if (s->iter) {
+    // IterativeSystem::sweep (iterative.rs:387-404): frequencies in list
+    // order, the first failure reported with its index.
+    IterativeSystem &it = *s->iter;
+    it.reset_counters();
+    it.set_profiling(s->profiling);
+    int32_t it_max = 0;
+    int64_t it_total = 0;
+    double res_max = 0.0;
+    std::vector<size_t> iters;
+    for (size_t k = 0; k < K; ++k) {
+      fh_status st = it.impedance(freqs[k], z_out + 2 * p * p * k, iters, res_max);
+      if (st == FH_E_NOT_CONVERGED) {
+        ErrorState &e = error_state();  // index1 = port, value = residual, code = iterations
+        e.index0 = static_cast<int64_t>(k);
+        e.message += " (frequency index " + std::to_string(k) + ")";
+        return st;
+      }
+      if (st) return st;
+      for (size_t q = 0; q < iters.size(); ++q) {
+        it_max = std::max<int32_t>(it_max, static_cast<int32_t>(iters[q]));
+        it_total += static_cast<int64_t>(iters[q]);
+      }
+      for (size_t e = 0; e < 2 * p * p; ++e)
+        if (!std::isfinite(z_out[2 * p * p * k + e]))
+          return set_error(FH_E_SINGULAR,
+                           "the mesh system is singular or ill-conditioned at " + std::to_string(freqs[k]) + " Hz",
+                           static_cast<int64_t>(k), 0, freqs[k]);
+    }
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->nodes = s->counts[0];
+      info->segments = s->counts[1];
+      info->filaments = s->counts[2];
+      info->meshes = s->counts[3];
+      info->internal_meshes = s->counts[4];
+      info->ports = s->counts[5];
+      info->inductance_s = s->inductance_s;
+      info->assembly_s = s->assembly_s;
+      info->solve_s = seconds_since(t0);
+      info->solver = FH_SOLVER_ITERATIVE;
+      info->gmres_iterations_max = it_max;
+      info->gmres_iterations = it_total;
+      info->gmres_residual_max = res_max;
+      info->operator_applies = static_cast<int64_t>(it.applies());
+      info->operator_apply_ms = s->profiling ? it.apply_ms() : 0.0;
+    }
+    return FH_OK;
+  }
   Profiler prof;
   Profiler *pp = s->profiling ? &prof : nullptr;
   int refine_steps_max = 0;
@@ -420,6 +554,12 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used) : 0;
     info->update_kernel_flops = pp ? prof.flops : 0.0;
     info->update_kernel_unit_flops = pp ? prof.unit_flops : 0.0;
+    info->solver = FH_SOLVER_DENSE;
+    info->gmres_iterations_max = 0;
+    info->gmres_iterations = 0;
+    info->gmres_residual_max = 0.0;
+    info->operator_applies = 0;
+    info->operator_apply_ms = 0.0;
   }
   return FH_OK;
 }
@@ -462,6 +602,69 @@ fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *
   if (methods)
     FH_HIP_TRY(hipMemcpyAsync(methods, d_m.ptr, n * n, hipMemcpyDeviceToHost, stream));
   FH_HIP_TRY(hipStreamSynchronize(stream));
+  return FH_OK;
+}
+
+fh_status fh_pfft_create(const fh_filament *filaments, size_t n, const fh_pfft_params *params,
+                         fh_pfft **out) {
+  clear_error();
+  if (!params || !out || (n && !filaments)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  fh_status st = check_device();
+  if (st) return st;
+  std::unique_ptr<fh_pfft> op(new fh_pfft());
+  FH_HIP_TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+  std::vector<fh_filament> f(filaments, filaments + n);
+  if ((st = op->op.build(f, *params, op->stream))) return st;
+  *out = op.release();
+  return FH_OK;
+}
+
+fh_status fh_pfft_apply(fh_pfft *op, const double *x, double *y, size_t nvec) {
+  clear_error();
+  if (!op) return set_error(FH_E_INVALID_ARGUMENT, "null operator");
+  const size_t n = op->op.size();
+  if (n == 0 || nvec == 0) return FH_OK;
+  if (!x || !y) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (nvec > 1024) return set_error(FH_E_INVALID_ARGUMENT, "at most 1024 vectors per call");
+  fh_status st;
+  if (op->dx.count < n * nvec && ((st = op->dx.allocate(n * nvec)) || (st = op->dy.allocate(n * nvec))))
+    return st;
+  FH_HIP_TRY(hipMemcpyAsync(op->dx.ptr, x, n * nvec * sizeof(double), hipMemcpyHostToDevice, op->stream));
+  if ((st = op->op.apply(op->dx.ptr, op->dy.ptr, static_cast<int>(nvec), op->stream))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(y, op->dy.ptr, n * nvec * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+  FH_HIP_TRY(hipStreamSynchronize(op->stream));
+  return FH_OK;
+}
+
+fh_status fh_pfft_get_stats(const fh_pfft *op, fh_pfft_stats *out) {
+  clear_error();
+  if (!op || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  *out = op->op.stats();
+  return FH_OK;
+}
+
+fh_status fh_pfft_data(const fh_pfft *op, int64_t *proj_off, uint32_t *proj_idx, double *proj_w,
+                       int64_t *near_off, uint32_t *near_col, double *near_val) {
+  clear_error();
+  if (!op) return set_error(FH_E_INVALID_ARGUMENT, "null operator");
+  return op->op.copy_data(proj_off, proj_idx, proj_w, near_off, near_col, near_val);
+}
+
+void fh_pfft_destroy(fh_pfft *op) { delete op; }
+
+fh_status fh_gmres_dense(size_t n, const double *a, const double *b, double *x,
+                         const fh_gmres_params *params, int32_t complex_arith,
+                         uint64_t *iterations, double *relative_residual, int32_t *converged) {
+  clear_error();
+  if (!params || (n && (!a || !b || !x))) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  fh_status st = check_device();
+  if (st) return st;
+  GmresOutcome o;
+  if ((st = gmres_dense(n, a, b, x, *params, complex_arith != 0, o))) return st;
+  if (iterations) *iterations = o.iterations;
+  if (relative_residual) *relative_residual = o.relative_residual;
+  if (converged) *converged = o.converged ? 1 : 0;
   return FH_OK;
 }
 

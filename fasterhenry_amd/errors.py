@@ -55,3 +55,9 @@ class HipError(SolveError):
 
 class InvalidArgument(SolveError):
     """A null pointer, a size out of range or an unsupported option."""
+
+
+class PfftError(SolveError):
+    """The pFFT operator cannot be built (pfft/mod.rs:236-251): code 1 is
+    InvalidParameter, code 2 GridTooLarge (index0 = padded points, index1 =
+    the limit)."""

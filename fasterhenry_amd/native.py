@@ -27,14 +27,16 @@ SYMBOLS = [
     "fh_system_set_profiling", "fh_system_counts", "fh_system_filaments",
     "fh_system_inductance", "fh_system_blocks", "fh_system_destroy", "fh_solve",
     "fh_partial_inductance", "fh_discretize", "fh_device_math", "fh_last_error",
-    "fh_error_detail", "fh_version",
+    "fh_error_detail", "fh_version", "fh_iterative_default_params", "fh_system_create_iterative",
+    "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
+    "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense",
 ]
 
 _ERRORS = {
     1: errors.InvalidArgument, 2: errors.GeometryError, 3: errors.DiscretizeError,
     4: errors.KernelError, 5: errors.MeshError, 6: errors.InvalidGrading,
     7: errors.InvalidFrequency, 8: errors.Singular, 9: errors.NotConverged,
-    10: errors.SolveError, 11: errors.SolveError, 12: errors.HipError,
+    10: errors.SolveError, 11: errors.SolveError, 12: errors.HipError, 13: errors.PfftError,
 }
 
 
@@ -51,10 +53,45 @@ class SweepInfo(C.Structure):
                 ("solve_s", C.c_double), ("refine_steps_max", C.c_int32),
                 ("factor_arith", C.c_int32), ("refine_change_max", C.c_double),
                 ("update_kernel_ms", C.c_double), ("update_kernel_launches", C.c_int64),
-                ("update_kernel_flops", C.c_double), ("update_kernel_unit_flops", C.c_double)]
+                ("update_kernel_flops", C.c_double), ("update_kernel_unit_flops", C.c_double),
+                ("solver", C.c_int32), ("gmres_iterations_max", C.c_int32),
+                ("gmres_iterations", C.c_int64), ("gmres_residual_max", C.c_double),
+                ("operator_applies", C.c_int64), ("operator_apply_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PfftParams(C.Structure):
+    """fh_pfft_params: PfftParams (pfft/mod.rs:183-231)."""
+    _fields_ = [("auto_grid", C.c_int32), ("interpolation_order", C.c_int32),
+                ("cells_per_filament", C.c_double), ("spacing", C.c_double),
+                ("near_field_radius", C.c_double)]
+
+
+class GmresParams(C.Structure):
+    """fh_gmres_params: GmresParams (gmres.rs:29-60)."""
+    _fields_ = [("tolerance", C.c_double), ("restart", C.c_uint64),
+                ("max_iterations", C.c_uint64)]
+
+
+class IterativeParams(C.Structure):
+    """fh_iterative_params: IterativeParams (iterative.rs:98-105)."""
+    _fields_ = [("pfft", PfftParams), ("gmres", GmresParams)]
+
+
+class PfftStats(C.Structure):
+    """fh_pfft_stats: PfftStats (pfft/mod.rs:255-270)."""
+    _fields_ = [("filaments", C.c_uint64), ("grid_spacing", C.c_double),
+                ("grid_dims", C.c_uint64 * 3), ("fft_dims", C.c_uint64 * 3),
+                ("projection_entries", C.c_uint64), ("near_entries", C.c_uint64),
+                ("memory_bytes", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {"filaments": self.filaments, "grid_spacing": self.grid_spacing,
+                "grid_dims": list(self.grid_dims), "fft_dims": list(self.fft_dims),
+                "projection_entries": self.projection_entries,
+                "near_entries": self.near_entries, "memory_bytes": self.memory_bytes}
 
 
 _lib = None
@@ -108,6 +145,28 @@ def lib() -> C.CDLL:
         L.fh_error_detail.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64), dp,
                                       C.POINTER(C.c_int32)]
         L.fh_version.restype = C.c_char_p
+        L.fh_iterative_default_params.restype = None
+        L.fh_iterative_default_params.argtypes = [C.POINTER(IterativeParams)]
+        L.fh_system_create_iterative.restype = st
+        L.fh_system_create_iterative.argtypes = [C.POINTER(Problem), C.POINTER(IterativeParams),
+                                                 C.POINTER(vp)]
+        L.fh_system_operator_stats.restype = st
+        L.fh_system_operator_stats.argtypes = [vp, C.POINTER(PfftStats)]
+        L.fh_pfft_default_params.restype = None
+        L.fh_pfft_default_params.argtypes = [C.POINTER(PfftParams)]
+        L.fh_pfft_create.restype = st
+        L.fh_pfft_create.argtypes = [vp, sz, C.POINTER(PfftParams), C.POINTER(vp)]
+        L.fh_pfft_apply.restype = st
+        L.fh_pfft_apply.argtypes = [vp, vp, vp, sz]
+        L.fh_pfft_get_stats.restype = st
+        L.fh_pfft_get_stats.argtypes = [vp, C.POINTER(PfftStats)]
+        L.fh_pfft_data.restype = st
+        L.fh_pfft_data.argtypes = [vp] * 7
+        L.fh_pfft_destroy.restype = None
+        L.fh_pfft_destroy.argtypes = [vp]
+        L.fh_gmres_dense.restype = st
+        L.fh_gmres_dense.argtypes = [sz, vp, vp, vp, C.POINTER(GmresParams), C.c_int32,
+                                     C.POINTER(C.c_uint64), dp, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 

@@ -1,7 +1,7 @@
 /*
  * fasterhenry_gpu.h -- C ABI of the MI355X (gfx950) extraction hot path.
  *
- * A drop-in for the dense solve path of fasterhenry 0.2.0.  The reference is
+ * A drop-in for the solve paths of fasterhenry 0.2.0.  The reference is
  * a safe Rust library without an FFI; each entry below names the reference
  * item it replaces (paths relative to the reference repository root).  The
  * ABI is plain C: fixed-width integers, doubles and pointers, caller-owned
@@ -39,10 +39,15 @@ typedef enum fh_status {
   FH_E_INVALID_GRADING = 6,  /* SolveError::InvalidGrading: index0 = segment */
   FH_E_INVALID_FREQUENCY = 7,/* SolveError::InvalidFrequency: index0, value */
   FH_E_SINGULAR = 8,         /* SolveError::Singular: index0 = frequency index, value = Hz */
-  FH_E_NOT_CONVERGED = 9,    /* refinement missed 1e-10: index0 = frequency index */
+  FH_E_NOT_CONVERGED = 9,    /* index0 = frequency index; refinement missed 1e-10
+                                (index1 = steps, value = Hz) or GMRES missed its
+                                tolerance (index1 = port, value = relative residual,
+                                code = iterations) */
   FH_E_UNSUPPORTED = 10,     /* a path this build does not provide */
   FH_E_NO_MEMORY = 11,       /* host or device allocation failed */
-  FH_E_HIP = 12              /* HIP runtime error */
+  FH_E_HIP = 12,             /* HIP runtime error */
+  FH_E_PFFT = 13             /* PfftError (pfft/mod.rs:236-251): code 1 = InvalidParameter,
+                                2 = GridTooLarge (index0 = padded points, index1 = limit) */
 } fh_status;
 
 /* SolverChoice (solve.rs:169-189) and Solver (solve.rs:143-158). */
@@ -54,6 +59,41 @@ typedef enum fh_solver { FH_SOLVER_DENSE = 0, FH_SOLVER_ITERATIVE = 1 } fh_solve
  * cores, FP64 iterative refinement of the solution to a 1e-10 relative
  * Schur-complement change, FP64 everywhere else. */
 typedef enum fh_precision { FH_FP64_PLAIN = 0, FH_FP32_REFINED = 1 } fh_precision;
+
+/* PfftParams (pfft/mod.rs:183-231).  auto_grid = 1: GridSpacing::Auto with
+ * cells_per_filament; 0: GridSpacing::Fixed(spacing) in metres.
+ * near_field_radius in grid spacings (>= interpolation_order + 1). */
+typedef struct fh_pfft_params {
+  int32_t auto_grid;
+  int32_t interpolation_order;
+  double cells_per_filament;
+  double spacing;
+  double near_field_radius;
+} fh_pfft_params;
+
+/* GmresParams (gmres.rs:29-60). */
+typedef struct fh_gmres_params {
+  double tolerance;
+  uint64_t restart;
+  uint64_t max_iterations;
+} fh_gmres_params;
+
+/* IterativeParams (iterative.rs:98-105). */
+typedef struct fh_iterative_params {
+  fh_pfft_params pfft;
+  fh_gmres_params gmres;
+} fh_iterative_params;
+
+/* PfftStats (pfft/mod.rs:255-270). */
+typedef struct fh_pfft_stats {
+  uint64_t filaments;
+  double grid_spacing;
+  uint64_t grid_dims[3];
+  uint64_t fft_dims[3];
+  uint64_t projection_entries;
+  uint64_t near_entries;
+  uint64_t memory_bytes; /* device memory held by the operator */
+} fh_pfft_stats;
 
 /* SolverChoice::resolve (solve.rs:179-189). */
 fh_solver fh_resolve(fh_solver_choice choice, size_t n_filaments);
@@ -121,19 +161,44 @@ typedef struct fh_sweep_info {
   int64_t update_kernel_launches;
   double update_kernel_flops;      /* algorithmic (FP64/FP32-equivalent) flops */
   double update_kernel_unit_flops; /* flops executed on the matrix-core units */
+  /* the path that produced the result (fh_solver) */
+  int32_t solver;
+  /* iterative path: most operator products of one port's GMRES solve, their
+   * total, the worst relative residual reached; with profiling, the time in
+   * operator products (each on the real and imaginary branch currents) */
+  int32_t gmres_iterations_max;
+  int64_t gmres_iterations;
+  double gmres_residual_max;
+  int64_t operator_applies;
+  double operator_apply_ms;
 } fh_sweep_info;
 
 typedef struct fh_system fh_system;
 
-/* MeshSystem::assemble (solve.rs:680-766): discretize, build the loop basis,
- * fill L on the GPU, reduce to mesh blocks on the GPU; the blocks stay
- * resident in device memory until fh_system_destroy.  Only the dense path
- * is provided: choice = FH_ITERATIVE, or FH_AUTO above
- * FH_DENSE_PATH_MAX_FILAMENTS, returns FH_E_UNSUPPORTED. */
+/* MeshSystem::assemble (solve.rs:680-766) or, when the choice resolves to
+ * the iterative path, IterativeSystem::assemble (iterative.rs:155-238) with
+ * the default parameters (solve.rs:931-938).  Dense: discretize, build the
+ * loop basis, fill L on the GPU, reduce to mesh blocks on the GPU; the
+ * blocks stay resident in device memory until fh_system_destroy.
+ * Iterative: the pFFT operator, port columns and Jacobi diagonals. */
 fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
                            fh_precision precision, fh_system **out);
 
-/* MeshSystem::sweep (solve.rs:881-894, 1039-1079): z_out receives
+/* IterativeParams::default (iterative.rs:98-105): PfftParams::default
+ * (pfft/mod.rs:213-231) and GmresParams::default (gmres.rs:48-60). */
+void fh_iterative_default_params(fh_iterative_params *out);
+
+/* IterativeSystem::assemble (iterative.rs:155-238) with explicit parameters.
+ * FH_E_PFFT if the operator cannot be built from params->pfft. */
+fh_status fh_system_create_iterative(const fh_problem *problem, const fh_iterative_params *params,
+                                     fh_system **out);
+
+/* IterativeSystem::operator_stats (iterative.rs:255-258); FH_E_UNSUPPORTED
+ * for a dense system. */
+fh_status fh_system_operator_stats(const fh_system *system, fh_pfft_stats *out);
+
+/* MeshSystem::sweep (solve.rs:881-894, 1039-1079) or IterativeSystem::sweep
+ * (iterative.rs:387-404): z_out receives
  * K x p x p complex values, row-major, each as {re, im}.  f = 0 is solved
  * in real arithmetic with Im Z exactly 0 (solve.rs:809-812).  The first
  * failing frequency in list order is reported.  info may be NULL. */
@@ -143,7 +208,8 @@ fh_status fh_sweep(fh_system *system, const double *freqs_hz, size_t K, double *
 /* MeshSystem::impedance (solve.rs:806-830) at one frequency. */
 fh_status fh_impedance(fh_system *system, double freq_hz, double *z_out);
 
-/* Changes the factorization arithmetic of an assembled system. */
+/* Changes the factorization arithmetic of an assembled dense system (no
+ * effect on an iterative one). */
 fh_status fh_system_set_precision(fh_system *system, fh_precision precision);
 
 /* Brackets each launch of the dominant kernel with HIP events on the
@@ -158,6 +224,7 @@ fh_status fh_system_counts(const fh_system *system, size_t counts[6]);
  * filaments: n; L: n x n row-major; mesh blocks row-major: l_ee m x m,
  * l_ep m x p, l_pp p x p, r_ee m x m, r_ep m x p, r_pp p x p. */
 fh_status fh_system_filaments(const fh_system *system, fh_filament *out);
+/* Dense systems only (FH_E_UNSUPPORTED for an iterative one). */
 fh_status fh_system_inductance(const fh_system *system, double *L_out);
 fh_status fh_system_blocks(const fh_system *system, double *r_ee, double *l_ee, double *r_ep,
                            double *l_ep, double *r_pp, double *l_pp);
@@ -179,6 +246,44 @@ fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *
  * filament count; out (capacity entries) may be NULL to query the count. */
 fh_status fh_discretize(const fh_problem *problem, fh_filament *out, size_t capacity,
                         size_t *n_out);
+
+/* The precorrected-FFT operator on its own: PfftOperator (pfft/mod.rs:270-528). */
+typedef struct fh_pfft fh_pfft;
+
+/* PfftParams::default (pfft/mod.rs:213-231). */
+void fh_pfft_default_params(fh_pfft_params *out);
+
+/* PfftOperator::new (pfft/mod.rs:332-445) for host filaments.  FH_E_PFFT
+ * (code 1 InvalidParameter, 2 GridTooLarge), FH_E_KERNEL for a near pair
+ * whose exact partial inductance fails (index0, index1). */
+fh_status fh_pfft_create(const fh_filament *filaments, size_t n, const fh_pfft_params *params,
+                         fh_pfft **out);
+
+/* PfftOperator::apply (pfft/mod.rs:482-528): y = L x for nvec host vectors
+ * of n values each, stored one after another. */
+fh_status fh_pfft_apply(fh_pfft *op, const double *x, double *y, size_t nvec);
+
+/* PfftOperator::stats (pfft/mod.rs:255-270). */
+fh_status fh_pfft_get_stats(const fh_pfft *op, fh_pfft_stats *out);
+
+/* Copies of the operator's sparse data, for parity checks; any pointer may
+ * be NULL.  Projection CSR (proj_off n + 1, proj_idx / proj_w
+ * projection_entries, grid points ascending within a row) and near-field
+ * corrections (near_off n + 1, near_col / near_val near_entries, full
+ * rows: partners j >= i in ascending order, mirrored). */
+fh_status fh_pfft_data(const fh_pfft *op, int64_t *proj_off, uint32_t *proj_idx, double *proj_w,
+                       int64_t *near_off, uint32_t *near_col, double *near_val);
+
+void fh_pfft_destroy(fh_pfft *op);
+
+/* gmres (gmres.rs:91-206) with the identity preconditioner on a dense
+ * complex matrix: a is n x n column-major {re, im}, b and x (initial guess
+ * in, solution out) n {re, im}.  complex_arith = 0 runs the real form on the
+ * real parts (imaginary parts must be zero).  Outcome (gmres.rs:63-73):
+ * iterations, relative residual, converged. */
+fh_status fh_gmres_dense(size_t n, const double *a, const double *b, double *x,
+                         const fh_gmres_params *params, int32_t complex_arith,
+                         uint64_t *iterations, double *relative_residual, int32_t *converged);
 
 /* Device-side log and atan of the fill kernels, for bit-level checks. */
 fh_status fh_device_math(const double *x, size_t n, double *log_out, double *atan_out);
