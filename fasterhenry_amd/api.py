@@ -200,9 +200,13 @@ class MeshSystem:
 
 
 def pfft_params(spacing: Optional[float] = None, cells_per_filament: float = 8.0,
-                near_field_radius: float = 8.0, interpolation_order: int = 5) -> native.PfftParams:
+                near_field_radius: float = 8.0, interpolation_order: int = 5,
+                fft_precision: str = "fp64") -> native.PfftParams:
     """PfftParams (pfft/mod.rs:183-231): GridSpacing::Fixed(spacing) when
-    spacing is given, else GridSpacing::Auto{cells_per_filament}."""
+    spacing is given, else GridSpacing::Auto{cells_per_filament}.
+    fft_precision "fp32" runs the far-field FFT convolution in FP32."""
+    if fft_precision not in ("fp64", "fp32"):
+        raise InvalidArgument(f"fft_precision must be 'fp64' or 'fp32', got {fft_precision!r}")
     p = native.PfftParams()
     native.lib().fh_pfft_default_params(C.byref(p))
     p.auto_grid = 0 if spacing is not None else 1
@@ -210,6 +214,7 @@ def pfft_params(spacing: Optional[float] = None, cells_per_filament: float = 8.0
     p.cells_per_filament = float(cells_per_filament)
     p.near_field_radius = float(near_field_radius)
     p.interpolation_order = int(interpolation_order)
+    p.fft_precision = 1 if fft_precision == "fp32" else 0
     return p
 
 

@@ -35,6 +35,7 @@
 #include <thread>
 #include <vector>
 
+#include "fh_fft.hpp"
 #include "fh_pfft.hpp"
 
 namespace fh {
@@ -168,11 +169,6 @@ __host__ __device__ inline void stencil_weights(int m, double t, double *w) {
   }
 }
 
-// near.rs:131-139
-__host__ __device__ inline double grid_kernel(int64_t dx, int64_t dy, int64_t dz, double h) {
-  const double r2 = static_cast<double>(dx * dx + dy * dy + dz * dz);
-  return r2 == 0.0 ? 0.0 : 1.0 / (h * sqrt(r2));
-}
 
 // Projection of one filament (grid.rs:155-216): the Gauss nodes along the
 // filament's three axes, the box of grid points their stencils cover.
@@ -289,14 +285,76 @@ __device__ inline void grid_point(int64_t gidx, const GridDesc &g, int64_t p[3])
   p[0] = rest / g.dims[1];
 }
 
+// K(d) for every offset a row's evaluation can meet, |d_a| <= r_a, computed
+// by pfft_grid_kernel itself (so a lookup equals the direct evaluation).
+struct KernelTable {
+  const double *t;
+  int32_t r0, r1, r2, d1, d2;
+};
+
+__global__ void kernel_table_kernel(double *__restrict__ t, int32_t r0, int32_t r1, int32_t r2, double h) {
+  const int64_t d0 = 2 * r0 + 1, d1 = 2 * r1 + 1, d2 = 2 * r2 + 1;
+  const int64_t total = d0 * d1 * d2;
+  for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < total;
+       q += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t z = q % d2, y = (q / d2) % d1, x = q / (d2 * d1);
+    t[q] = pfft_grid_kernel(x - r0, y - r1, z - r2, h);
+  }
+}
+
+template <bool TABLE>
+__device__ inline double kval(const KernelTable &T, const GridDesc &g, int dx, int dy, int dz) {
+  if (TABLE) return T.t[(static_cast<int64_t>(dx + T.r0) * T.d1 + (dy + T.r1)) * T.d2 + (dz + T.r2)];
+  return pfft_grid_kernel(dx, dy, dz, g.h);
+}
+
+constexpr int kSrcLds = 1024;  // source entries of a row staged in LDS
+
+// Row i's projection entries as grid coordinates and weights, in entry
+// order, in LDS (rows with more than kSrcLds entries read them from memory).
+struct SrcStage {
+  int x[kSrcLds], y[kSrcLds], z[kSrcLds];
+  double w[kSrcLds];
+};
+
+template <bool STAGED>
+__device__ inline void source(const SrcStage &S, int64_t s0, int s, const uint32_t *proj_idx,
+                              const double *proj_w, const GridDesc &g, int &x, int &y, int &z, double &w) {
+  if (STAGED) {
+    x = S.x[s];
+    y = S.y[s];
+    z = S.z[s];
+    w = S.w[s];
+  } else {
+    int64_t p[3];
+    grid_point(proj_idx[s0 + s], g, p);
+    x = static_cast<int>(p[0]);
+    y = static_cast<int>(p[1]);
+    z = static_cast<int>(p[2]);
+    w = proj_w[s0 + s];
+  }
+}
+
+__device__ inline void stage_sources(SrcStage &S, int64_t s0, int ns, const uint32_t *proj_idx,
+                                     const double *proj_w, const GridDesc &g) {
+  for (int s = threadIdx.x; s < ns && s < kSrcLds; s += blockDim.x) {
+    int64_t p[3];
+    grid_point(proj_idx[s0 + s], g, p);
+    S.x[s] = static_cast<int>(p[0]);
+    S.y[s] = static_cast<int>(p[1]);
+    S.z[s] = static_cast<int>(p[2]);
+    S.w[s] = proj_w[s0 + s];
+  }
+  __syncthreads();
+}
+
 // Direct rows: one thread per partner, the double sum in the reference's
 // order (partner entries outer, source entries inner).
-__global__ void __launch_bounds__(256) interactions_direct_kernel(
-    const RowTask *__restrict__ tasks, const int64_t *__restrict__ proj_off,
-    const uint32_t *__restrict__ proj_idx, const double *__restrict__ proj_w,
-    const int64_t *__restrict__ partner_idx, GridDesc g, double *__restrict__ approx) {
-  const RowTask t = tasks[blockIdx.x];
-  const int64_t s0 = proj_off[t.row], s1 = proj_off[t.row + 1];
+template <bool STAGED, bool TABLE>
+__device__ void direct_row(const RowTask &t, const SrcStage &S, int64_t s0, int ns,
+                           const int64_t *__restrict__ proj_off, const uint32_t *__restrict__ proj_idx,
+                           const double *__restrict__ proj_w, const int64_t *__restrict__ partner_idx,
+                           const GridDesc &g, const KernelTable &T, double *__restrict__ approx) {
   for (int q = threadIdx.x; q < t.partners; q += blockDim.x) {
     const int64_t j = partner_idx[t.partner_off + q];
     double total = 0.0;
@@ -304,10 +362,12 @@ __global__ void __launch_bounds__(256) interactions_direct_kernel(
       int64_t hp[3];
       grid_point(proj_idx[e], g, hp);
       double potential = 0.0;
-      for (int64_t s = s0; s < s1; ++s) {
-        int64_t gp[3];
-        grid_point(proj_idx[s], g, gp);
-        potential += proj_w[s] * grid_kernel(hp[0] - gp[0], hp[1] - gp[1], hp[2] - gp[2], g.h);
+      for (int s = 0; s < ns; ++s) {
+        int x, y, z;
+        double w;
+        source<STAGED>(S, s0, s, proj_idx, proj_w, g, x, y, z, w);
+        potential += w * kval<TABLE>(T, g, static_cast<int>(hp[0]) - x, static_cast<int>(hp[1]) - y,
+                                     static_cast<int>(hp[2]) - z);
       }
       total += proj_w[e] * potential;
     }
@@ -315,42 +375,73 @@ __global__ void __launch_bounds__(256) interactions_direct_kernel(
   }
 }
 
+template <bool TABLE>
+__global__ void __launch_bounds__(256) interactions_direct_kernel(
+    const RowTask *__restrict__ tasks, const int64_t *__restrict__ proj_off,
+    const uint32_t *__restrict__ proj_idx, const double *__restrict__ proj_w,
+    const int64_t *__restrict__ partner_idx, GridDesc g, KernelTable T, double *__restrict__ approx) {
+  __shared__ SrcStage S;
+  const RowTask t = tasks[blockIdx.x];
+  const int64_t s0 = proj_off[t.row];
+  const int ns = static_cast<int>(proj_off[t.row + 1] - s0);
+  stage_sources(S, s0, ns, proj_idx, proj_w, g);
+  if (ns <= kSrcLds) direct_row<true, TABLE>(t, S, s0, ns, proj_off, proj_idx, proj_w, partner_idx, g, T, approx);
+  else direct_row<false, TABLE>(t, S, s0, ns, proj_off, proj_idx, proj_w, partner_idx, g, T, approx);
+}
+
 // Box rows: the potential psi of row i's projection over the box of its
 // partners' entries (each cell sums the sources in order; fused multiply-add
 // for the first bz - bz % 4 cells of a z-row, as near.rs:189-207 does in
 // blocks of four, plain multiply-add for the tail), held in LDS when it fits
 // (`lds_cells`), else in `scratch`; then one thread per partner.
-__global__ void __launch_bounds__(256) interactions_box_kernel(
-    const RowTask *__restrict__ tasks, const int64_t *__restrict__ proj_off,
-    const uint32_t *__restrict__ proj_idx, const double *__restrict__ proj_w,
-    const int64_t *__restrict__ partner_idx, GridDesc g, int lds_cells,
-    double *__restrict__ scratch, double *__restrict__ approx) {
-  extern __shared__ double psi_lds[];
-  const RowTask t = tasks[blockIdx.x];
-  const int64_t s0 = proj_off[t.row], s1 = proj_off[t.row + 1];
+template <bool STAGED, bool TABLE>
+__device__ void box_potential(const RowTask &t, const SrcStage &S, int64_t s0, int ns,
+                              const uint32_t *__restrict__ proj_idx, const double *__restrict__ proj_w,
+                              const GridDesc &g, const KernelTable &T, double *psi) {
   const int bx = t.box_dims[0], by = t.box_dims[1], bz = t.box_dims[2];
   const int box_len = bx * by * bz;
   const int body = bz - bz % 4;
-  double *psi = box_len <= lds_cells ? psi_lds : scratch + t.scratch_off;
   for (int cell = threadIdx.x; cell < box_len; cell += blockDim.x) {
     const int z = cell % bz, y = (cell / bz) % by, x = cell / (bz * by);
-    const int64_t px = t.box_lo[0] + x, py = t.box_lo[1] + y, pz = t.box_lo[2] + z;
+    const int px = static_cast<int>(t.box_lo[0]) + x, py = static_cast<int>(t.box_lo[1]) + y,
+              pz = static_cast<int>(t.box_lo[2]) + z;
     double acc = 0.0;
     if (z < body) {
-      for (int64_t s = s0; s < s1; ++s) {
-        int64_t gp[3];
-        grid_point(proj_idx[s], g, gp);
-        acc = __builtin_fma(proj_w[s], grid_kernel(px - gp[0], py - gp[1], pz - gp[2], g.h), acc);
+      for (int s = 0; s < ns; ++s) {
+        int sx, sy, sz;
+        double w;
+        source<STAGED>(S, s0, s, proj_idx, proj_w, g, sx, sy, sz, w);
+        acc = __builtin_fma(w, kval<TABLE>(T, g, px - sx, py - sy, pz - sz), acc);
       }
     } else {
-      for (int64_t s = s0; s < s1; ++s) {
-        int64_t gp[3];
-        grid_point(proj_idx[s], g, gp);
-        acc += proj_w[s] * grid_kernel(px - gp[0], py - gp[1], pz - gp[2], g.h);
+      for (int s = 0; s < ns; ++s) {
+        int sx, sy, sz;
+        double w;
+        source<STAGED>(S, s0, s, proj_idx, proj_w, g, sx, sy, sz, w);
+        acc += w * kval<TABLE>(T, g, px - sx, py - sy, pz - sz);
       }
     }
     psi[cell] = acc;
   }
+}
+
+template <bool TABLE>
+__global__ void __launch_bounds__(512) interactions_box_kernel(
+    const RowTask *__restrict__ tasks, const int64_t *__restrict__ proj_off,
+    const uint32_t *__restrict__ proj_idx, const double *__restrict__ proj_w,
+    const int64_t *__restrict__ partner_idx, GridDesc g, KernelTable T, int lds_cells,
+    double *__restrict__ scratch, double *__restrict__ approx) {
+  extern __shared__ double psi_lds[];
+  __shared__ SrcStage S;
+  const RowTask t = tasks[blockIdx.x];
+  const int64_t s0 = proj_off[t.row];
+  const int ns = static_cast<int>(proj_off[t.row + 1] - s0);
+  const int by = t.box_dims[1], bz = t.box_dims[2];
+  const int box_len = t.box_dims[0] * by * bz;
+  double *psi = box_len <= lds_cells ? psi_lds : scratch + t.scratch_off;
+  stage_sources(S, s0, ns, proj_idx, proj_w, g);
+  if (ns <= kSrcLds) box_potential<true, TABLE>(t, S, s0, ns, proj_idx, proj_w, g, T, psi);
+  else box_potential<false, TABLE>(t, S, s0, ns, proj_idx, proj_w, g, T, psi);
   __syncthreads();
   for (int q = threadIdx.x; q < t.partners; q += blockDim.x) {
     const int64_t j = partner_idx[t.partner_off + q];
@@ -382,7 +473,7 @@ __global__ void kernel_grid_kernel(double2 *__restrict__ data, int64_t n0, int64
     };
     bool a, b, c;
     const int64_t di = off(i, n0, m0, a), dj = off(j, n1, m1, b), dk = off(k, n2, m2, c);
-    data[q] = make_double2(a && b && c ? grid_kernel(di, dj, dk, h) : 0.0, 0.0);
+    data[q] = make_double2(a && b && c ? pfft_grid_kernel(di, dj, dk, h) : 0.0, 0.0);
   }
 }
 
@@ -481,6 +572,80 @@ __global__ void zero_kernel(double *__restrict__ p, int64_t count) {
   if (q < count) p[q] = 0.0;
 }
 
+// Up to 8 sources (vector, direction component) of one convolution batch,
+// paired into channels (2c, 2c + 1).
+constexpr int kBatchSources = 8;
+struct SourceSet {
+  int count;
+  int vec[kBatchSources];
+  int comp[kBatchSources];
+};
+
+__device__ inline float2 to_c(double a, double b, float2) { return make_float2(static_cast<float>(a), static_cast<float>(b)); }
+__device__ inline double2 to_c(double a, double b, double2) { return make_double2(a, b); }
+
+// Sources on the compact grid: q_s(g) = sum over the transposed projection
+// (filaments in order) of (w l_c) x_v, in FP64, stored as channel values.
+template <typename T2>
+__global__ void __launch_bounds__(256) spread_compact_kernel(
+    T2 *__restrict__ grid, const int64_t *__restrict__ soff, const uint32_t *__restrict__ sfil,
+    const double *__restrict__ sw, const double *__restrict__ dirs, const double *__restrict__ x,
+    int64_t n, int64_t G, SourceSet ss) {
+  const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (g >= G) return;
+  double acc[kBatchSources];
+#pragma unroll
+  for (int s = 0; s < kBatchSources; ++s) acc[s] = 0.0;
+  for (int64_t e = soff[g]; e < soff[g + 1]; ++e) {
+    const int64_t f = sfil[e];
+    const double w = sw[e];
+#pragma unroll
+    for (int s = 0; s < kBatchSources; ++s)
+      if (s < ss.count) acc[s] += w * dirs[3 * f + ss.comp[s]] * x[ss.vec[s] * n + f];
+  }
+#pragma unroll
+  for (int c = 0; c < kBatchSources / 2; ++c)
+    if (2 * c < ss.count) grid[c * G + g] = to_c(acc[2 * c], acc[2 * c + 1], T2{});
+}
+
+// far_v[i] += l_c(i) sum_e W_e phi_s(g_e) for the batch's sources (the
+// components of one vector in ascending order): one wavefront per filament,
+// lanes striding its projection entries, partial sums combined in a fixed
+// butterfly order.
+template <typename T2>
+__global__ void __launch_bounds__(256) gather_compact_kernel(
+    const T2 *__restrict__ grid, const int64_t *__restrict__ poff, const uint32_t *__restrict__ pidx,
+    const double *__restrict__ pw, const double *__restrict__ dirs, int64_t n, int64_t G, SourceSet ss,
+    double *__restrict__ far) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  double sum[kBatchSources];
+#pragma unroll
+  for (int s = 0; s < kBatchSources; ++s) sum[s] = 0.0;
+  const int C = (ss.count + 1) / 2;
+  for (int64_t e = poff[i] + lane; e < poff[i + 1]; e += 64) {
+    const int64_t g = pidx[e];
+    const double w = pw[e];
+#pragma unroll
+    for (int c = 0; c < kBatchSources / 2; ++c)
+      if (c < C) {
+        const T2 v = grid[c * G + g];
+        sum[2 * c] += w * static_cast<double>(v.x);
+        sum[2 * c + 1] += w * static_cast<double>(v.y);
+      }
+  }
+#pragma unroll
+  for (int s = 0; s < kBatchSources; ++s)
+    if (s < ss.count)
+      for (int d = 32; d > 0; d >>= 1) sum[s] += __shfl_xor(sum[s], d);
+  if (lane == 0) {
+#pragma unroll
+    for (int s = 0; s < kBatchSources; ++s)
+      if (s < ss.count) far[ss.vec[s] * n + i] += dirs[3 * i + ss.comp[s]] * sum[s];
+  }
+}
+
 unsigned blocks_for(int64_t total) {
   return static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 1 << 20));
 }
@@ -533,6 +698,9 @@ struct PfftImpl {
   rocfft_plan fwd = nullptr, inv = nullptr;
   rocfft_execution_info info = nullptr;
   size_t far_cap = 0;
+  PrunedConvolver conv;
+  bool use_conv = false;
+  int fft_kind = 0;
 
   ~PfftImpl() {
     if (fwd) rocfft_plan_destroy(fwd);
@@ -958,6 +1126,20 @@ fh_status PfftOperator::build(const std::vector<fh_filament> &F, const fh_pfft_p
     }
     box_tasks.push_back(t);
   }
+  // the offsets a row meets: its own entries against its partners' box
+  // (near.rs:305-315)
+  int64_t reach[3] = {0, 0, 0};
+  for (const std::vector<RowTask> *list : {&direct_tasks, &box_tasks})
+    for (const RowTask &t : *list) {
+      const size_t i = static_cast<size_t>(t.row);
+      for (int a = 0; a < 3; ++a) {
+        const int64_t plo = t.box_lo[a], phi_ = t.box_lo[a] + t.box_dims[a] - 1;
+        const int64_t r1 = phi_ - std::min(elo[3 * i + a], phi_), r2 = ehi[3 * i + a] - std::min(plo, ehi[3 * i + a]);
+        reach[a] = std::max(reach[a], std::max(r1, r2));
+      }
+    }
+  const int64_t table_len = (2 * reach[0] + 1) * (2 * reach[1] + 1) * (2 * reach[2] + 1);
+  const bool use_table = table_len <= (int64_t{1} << 26);
   std::vector<double> approx(P), exact(P);
   if (P > 0) {
     DeviceArray<fh_filament> d_f;
@@ -981,14 +1163,35 @@ fh_status PfftOperator::build(const std::vector<fh_filament> &F, const fh_pfft_p
       FH_HIP_TRY(hipMemcpyAsync(d_box.ptr, box_tasks.data(), box_tasks.size() * sizeof(RowTask),
                                 hipMemcpyHostToDevice, stream));
     if ((st = fill_pairs(d_f.ptr, n, d_pairs.ptr, P, d_exact.ptr, stream))) return st;
-    if (!direct_tasks.empty())
-      interactions_direct_kernel<<<static_cast<unsigned>(direct_tasks.size()), 256, 0, stream>>>(
-          d_direct.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, d_approx.ptr);
-    if (!box_tasks.empty())
-      interactions_box_kernel<<<static_cast<unsigned>(box_tasks.size()), 256,
-                                static_cast<size_t>(box_lds_cells) * sizeof(double), stream>>>(
-          d_box.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, box_lds_cells,
-          d_scratch.ptr, d_approx.ptr);
+    DeviceArray<double> d_table;
+    KernelTable T{nullptr, static_cast<int32_t>(reach[0]), static_cast<int32_t>(reach[1]),
+                  static_cast<int32_t>(reach[2]), static_cast<int32_t>(2 * reach[1] + 1),
+                  static_cast<int32_t>(2 * reach[2] + 1)};
+    if (use_table) {
+      if ((st = d_table.allocate(static_cast<size_t>(table_len)))) return st;
+      kernel_table_kernel<<<blocks_for(table_len), 256, 0, stream>>>(d_table.ptr, T.r0, T.r1, T.r2, h);
+      FH_HIP_TRY(hipGetLastError());
+      T.t = d_table.ptr;
+    }
+    if (!direct_tasks.empty()) {
+      if (use_table)
+        interactions_direct_kernel<true><<<static_cast<unsigned>(direct_tasks.size()), 256, 0, stream>>>(
+            d_direct.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, T, d_approx.ptr);
+      else
+        interactions_direct_kernel<false><<<static_cast<unsigned>(direct_tasks.size()), 256, 0, stream>>>(
+            d_direct.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, T, d_approx.ptr);
+    }
+    if (!box_tasks.empty()) {
+      const size_t lds = static_cast<size_t>(box_lds_cells) * sizeof(double);
+      if (use_table)
+        interactions_box_kernel<true><<<static_cast<unsigned>(box_tasks.size()), 512, lds, stream>>>(
+            d_box.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, T, box_lds_cells,
+            d_scratch.ptr, d_approx.ptr);
+      else
+        interactions_box_kernel<false><<<static_cast<unsigned>(box_tasks.size()), 512, lds, stream>>>(
+            d_box.ptr, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, d_pj.ptr, g, T, box_lds_cells,
+            d_scratch.ptr, d_approx.ptr);
+    }
     FH_HIP_TRY(hipGetLastError());
     FH_HIP_TRY(hipMemcpyAsync(exact.data(), d_exact.ptr, P * sizeof(double), hipMemcpyDeviceToHost, stream));
     FH_HIP_TRY(hipMemcpyAsync(approx.data(), d_approx.ptr, P * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1028,7 +1231,20 @@ fh_status PfftOperator::build(const std::vector<fh_filament> &F, const fh_pfft_p
   FH_HIP_TRY(hipMemcpyAsync(op.near_col.ptr, ncol.data(), NE * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
   FH_HIP_TRY(hipMemcpyAsync(op.near_val.ptr, nval.data(), NE * sizeof(double), hipMemcpyHostToDevice, stream));
 
-  // ---- convolver (fft.rs:91-140): rocFFT plans and the kernel's spectrum ----
+  // ---- convolver (fft.rs:91-140) ----
+  {
+    bool fits = false;
+    const bool single = prm.fft_precision == 1;
+    if ((st = op.conv.init(g.dims, op.padded, h, single, kBatchSources / 2, stream, fits))) return st;
+    if (fits) {
+      op.use_conv = true;
+      op.fft_kind = single ? 1 : 0;
+      FH_HIP_TRY(hipStreamSynchronize(stream));
+      return FH_OK;
+    }
+  }
+  // a padded axis too long for one LDS line: library 3-D FFT on the padded grid
+  op.fft_kind = 2;
   const int64_t M = op.padded_total();
   if ((st = op.buffer.allocate(static_cast<size_t>(M))) || (st = op.spectrum.allocate(static_cast<size_t>(M))))
     return st;
@@ -1085,6 +1301,40 @@ fh_status PfftOperator::apply(const double *d_x, double *d_y, int V, hipStream_t
   zero_kernel<<<blocks_for(static_cast<int64_t>(need)), 256, 0, stream>>>(op.far.ptr, static_cast<int64_t>(need));
   const int S = V * op.ncomp;  // sources (v, c), v outer
   const GridDesc &g = op.g;
+  if (op.use_conv) {
+    const int64_t G = op.grid_total();
+    for (int s0 = 0; s0 < S; s0 += kBatchSources) {
+      SourceSet ss{};
+      ss.count = std::min(kBatchSources, S - s0);
+      for (int q = 0; q < ss.count; ++q) {
+        ss.vec[q] = (s0 + q) / op.ncomp;
+        ss.comp[q] = op.comps[(s0 + q) % op.ncomp];
+      }
+      const int C = (ss.count + 1) / 2;
+      if (op.conv.single()) {
+        float2 *grid = static_cast<float2 *>(op.conv.grid());
+        spread_compact_kernel<float2><<<blocks_for(G), 256, 0, stream>>>(
+            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.dirs.ptr, d_x, n, G, ss);
+        FH_HIP_TRY(hipGetLastError());
+        if ((st = op.conv.convolve(C, stream))) return st;
+        gather_compact_kernel<float2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(
+            grid, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, op.dirs.ptr, n, G, ss, op.far.ptr);
+      } else {
+        double2 *grid = static_cast<double2 *>(op.conv.grid());
+        spread_compact_kernel<double2><<<blocks_for(G), 256, 0, stream>>>(
+            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.dirs.ptr, d_x, n, G, ss);
+        FH_HIP_TRY(hipGetLastError());
+        if ((st = op.conv.convolve(C, stream))) return st;
+        gather_compact_kernel<double2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(
+            grid, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, op.dirs.ptr, n, G, ss, op.far.ptr);
+      }
+      FH_HIP_TRY(hipGetLastError());
+    }
+    finish_kernel<<<blocks_for(static_cast<int64_t>(need)), 256, 0, stream>>>(d_y, op.far.ptr,
+                                                                               static_cast<int64_t>(need));
+    FH_HIP_TRY(hipGetLastError());
+    return FH_OK;
+  }
   const int64_t M = op.padded_total();
   for (int s = 0; s < S; s += 2) {
     const int va = s / op.ncomp, ca = op.comps[s % op.ncomp];
@@ -1121,7 +1371,10 @@ fh_pfft_stats PfftOperator::stats() const {
   const uint64_t grid = op.has_grid ? static_cast<uint64_t>(op.grid_total()) : 0;
   const uint64_t padded = op.has_grid ? static_cast<uint64_t>(op.padded_total()) : 0;
   s.memory_bytes = op.proj_entries * 2 * (4 + 8) + (op.n + 1 + grid + 1) * 8 + op.near_entries * 12 +
-                   (op.n + 1) * 8 + op.n * 24 + padded * 24 + op.far_cap * 8;
+                   (op.n + 1) * 8 + op.n * 24 + op.far_cap * 8 +
+                   (op.use_conv ? op.conv.bytes() : padded * 24);
+  s.fft_kind = op.fft_kind;
+  if (op.use_conv) op.conv.line_groups(s.line_group);
   return s;
 }
 

@@ -281,6 +281,8 @@ void fh_pfft_default_params(fh_pfft_params *out) {
   out->cells_per_filament = 8.0;  // pfft/mod.rs:213
   out->spacing = 0.0;
   out->near_field_radius = 8.0;   // pfft/mod.rs:215
+  out->fft_precision = 0;
+  out->reserved = 0;
 }
 
 fh_status fh_system_create_iterative(const fh_problem *problem, const fh_iterative_params *params,

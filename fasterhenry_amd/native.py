@@ -66,7 +66,8 @@ class PfftParams(C.Structure):
     """fh_pfft_params: PfftParams (pfft/mod.rs:183-231)."""
     _fields_ = [("auto_grid", C.c_int32), ("interpolation_order", C.c_int32),
                 ("cells_per_filament", C.c_double), ("spacing", C.c_double),
-                ("near_field_radius", C.c_double)]
+                ("near_field_radius", C.c_double), ("fft_precision", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class GmresParams(C.Structure):
@@ -85,13 +86,15 @@ class PfftStats(C.Structure):
     _fields_ = [("filaments", C.c_uint64), ("grid_spacing", C.c_double),
                 ("grid_dims", C.c_uint64 * 3), ("fft_dims", C.c_uint64 * 3),
                 ("projection_entries", C.c_uint64), ("near_entries", C.c_uint64),
-                ("memory_bytes", C.c_uint64)]
+                ("memory_bytes", C.c_uint64), ("fft_kind", C.c_int32),
+                ("line_group", C.c_int32 * 3)]
 
     def as_dict(self) -> dict:
         return {"filaments": self.filaments, "grid_spacing": self.grid_spacing,
                 "grid_dims": list(self.grid_dims), "fft_dims": list(self.fft_dims),
                 "projection_entries": self.projection_entries,
-                "near_entries": self.near_entries, "memory_bytes": self.memory_bytes}
+                "near_entries": self.near_entries, "memory_bytes": self.memory_bytes,
+                "fft_kind": self.fft_kind, "line_group": list(self.line_group)}
 
 
 _lib = None

@@ -62,13 +62,17 @@ typedef enum fh_precision { FH_FP64_PLAIN = 0, FH_FP32_REFINED = 1 } fh_precisio
 
 /* PfftParams (pfft/mod.rs:183-231).  auto_grid = 1: GridSpacing::Auto with
  * cells_per_filament; 0: GridSpacing::Fixed(spacing) in metres.
- * near_field_radius in grid spacings (>= interpolation_order + 1). */
+ * near_field_radius in grid spacings (>= interpolation_order + 1).
+ * fft_precision: arithmetic of the far-field FFT convolution, 0 = FP64 (the
+ * reference's), 1 = FP32; projection, near field and sums stay FP64. */
 typedef struct fh_pfft_params {
   int32_t auto_grid;
   int32_t interpolation_order;
   double cells_per_filament;
   double spacing;
   double near_field_radius;
+  int32_t fft_precision;
+  int32_t reserved;
 } fh_pfft_params;
 
 /* GmresParams (gmres.rs:29-60). */
@@ -93,6 +97,13 @@ typedef struct fh_pfft_stats {
   uint64_t projection_entries;
   uint64_t near_entries;
   uint64_t memory_bytes; /* device memory held by the operator */
+  /* the convolution used: 0 = FP64 pruned line FFTs, 1 = FP32 pruned line
+   * FFTs, 2 = FP64 library 3-D FFT (a padded axis too long for one line in
+   * LDS) */
+  int32_t fft_kind;
+  /* lines per workgroup the set-up autotuner chose for the axis-1 forward,
+   * axis-0 and axis-1 inverse line passes (0 when not tuned) */
+  int32_t line_group[3];
 } fh_pfft_stats;
 
 /* SolverChoice::resolve (solve.rs:179-189). */

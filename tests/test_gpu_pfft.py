@@ -91,6 +91,45 @@ def test_apply_matches_the_oracle_and_the_dense_fill(name):
     assert np.max(np.abs(y - dense)) / np.max(np.abs(dense)) < 1e-6
 
 
+@pytest.mark.parametrize("name", ["manhattan200", "skew200", "spiral"])
+def test_fp32_convolution_stays_within_the_reference_gate(name):
+    """FP32 far-field FFT: within the reference's 1e-6 of the dense fill and
+    within 1e-6 of the FP64 operator (its near field and projection are the
+    same FP64 data)."""
+    f = _sets()[name]
+    op64 = PfftOperator(f)
+    op32 = PfftOperator(f, pfft_params(fft_precision="fp32"))
+    assert op64.stats()["fft_kind"] == 0 and op32.stats()["fft_kind"] == 1
+    d64, d32 = op64.data(), op32.data()
+    assert np.array_equal(d64["near_val"], d32["near_val"])
+    assert np.array_equal(d64["proj_w"], d32["proj_w"])
+    x = _lcg_vector(len(f), 99)
+    y64, y32 = op64.apply(x), op32.apply(x)
+    assert np.linalg.norm(y32 - y64) / np.linalg.norm(y64) < 1e-6
+    dense = partial_inductance_matrix(f) @ x
+    assert np.linalg.norm(y32 - dense) / np.linalg.norm(dense) < 1e-6
+
+
+def test_long_axis_uses_the_library_fft_and_matches_the_oracle():
+    """A padded axis longer than one LDS line (two filaments 0.5 m apart on
+    a 0.1 mm grid: M0 = 10000) takes the library 3-D FFT."""
+    g = model.Geometry()
+    for x0 in (0.0, 0.5):
+        a = g.add_node(model.Node(x0, 0.0, 0.0))
+        b = g.add_node(model.Node(x0 + 1e-3, 0.0, 0.0))
+        g.add_segment(model.SegmentDef(a, b, 2e-4, 1e-4, 5.8e7))
+    f = oracle.discretize(model.pack(g, [], model.Discretization.uniform(1, 1)))
+    ref = oracle.Pfft(f, oracle.PfftParams.default(spacing=1e-4))
+    op = PfftOperator(f, pfft_params(spacing=1e-4))
+    st = op.stats()
+    assert st["fft_kind"] == 2 and st["fft_dims"][0] > 8191
+    x = np.array([1.0, -0.5])
+    y, yo = op.apply(x), ref.apply(x)
+    assert np.linalg.norm(y - yo) / np.linalg.norm(yo) < 1e-12
+    dense = partial_inductance_matrix(f) @ x
+    assert np.linalg.norm(y - dense) / np.linalg.norm(dense) < 1e-6
+
+
 def test_batched_apply_equals_single_applies():
     f = _random_set(200, False, 4)
     op = PfftOperator(f)
