@@ -58,6 +58,107 @@ def cpu_baseline(system, omega_hz: float, threads: int) -> dict:
     return {"seconds": dt, "z": z}
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def apply_bytes(st: dict, vectors: int, ncomp: int) -> float:
+    """HBM bytes of one operator product on `vectors` vectors with the
+    pruned FFT convolution: the transposed and direct projection CSR read
+    once each (u32 index + f64 weight), the compact grid and the pruned
+    intermediate arrays written and read once per line pass, the even
+    eighth of the spectrum, the near-field CSR, x and y."""
+    n, E, nnz = st["filaments"], st["projection_entries"], st["near_entries"]
+    N0, N1, N2 = st["grid_dims"]
+    M0, M1, M2 = st["fft_dims"]
+    c = (vectors * ncomp + 1) // 2               # complex channels
+    z = 8 if st["fft_kind"] == 1 else 16          # complex value
+    G = N0 * N1 * N2
+    a = c * N0 * N1 * M2 * z                      # after the axis-2 pass
+    b = c * N0 * M1 * M2 * z                      # after the axis-1 pass
+    spec = (M0 // 2 + 1) * (M1 // 2 + 1) * (M2 // 2 + 1) * (4 if st["fft_kind"] == 1 else 8)
+    spread = 12 * E + 8 * G + c * G * z
+    passes = (c * G * z + a) + (a + b) + (2 * b + spec) + (b + a) + (a + c * G * z)
+    gather = 12 * E + 8 * n + c * G * z
+    near = 12 * nnz + 8 * n + vectors * n * 8 * 3
+    return float(spread + passes + gather + near)
+
+
+def matrix_free(steps: int, warmup: int, dense_check=None) -> dict:
+    """The matrix-free sweep: meander(157) at 2 x 2 (99 224 filaments,
+    benches/scaling.rs:91), pFFT operator with the reference's default grid
+    (auto, 8 cells per filament, order 5, radius 8) and an FP32 FFT
+    convolution, FP64 near field and Arnoldi, GMRES restart 30, tolerance
+    1e-8, 8 log frequencies 1 MHz - 1 GHz.  A step is one sweep (assembly
+    timed once).  With dense_check = (packed, ports) of meander(49), the
+    same sweep's iterative Z at 9 800 filaments is compared with the dense
+    FP64 GPU solve."""
+    from fasterhenry_amd import (IterativeSystem, MeshSystem, fixtures, iterative_params, model,
+                                 pfft_params)
+
+    freqs = fixtures.log_frequencies(1e6, 1e9, 8)
+    params = iterative_params(pfft_params(fft_precision="fp32"), tolerance=1e-8, restart=30)
+    g, ports, d = fixtures.meander_problem(157)
+    packed = model.pack(g, ports, d)
+    print(f"# matrix-free: assembling meander(157): {packed.filament_count} filaments", flush=True)
+    t0 = time.perf_counter()
+    it = IterativeSystem.assemble_packed(packed, ports, params)
+    assembly_s = time.perf_counter() - t0
+    st = it.operator_stats()
+    counts = it.counts()
+    print(f"# matrix-free: assembled in {assembly_s:.2f} s, grid {st['grid_dims']}, "
+          f"padded {st['fft_dims']}", flush=True)
+    for w in range(warmup):
+        it.sweep_raw(freqs)
+    it.set_profiling(True)
+    infos = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        z, info = it.sweep_raw(freqs)
+        infos.append(info.as_dict())
+        print(f"# matrix-free step {k + 1}/{steps}: {info.solve_s:.3f} s, "
+              f"{info.gmres_iterations} GMRES iterations", flush=True)
+    elapsed = time.perf_counter() - t0
+    it.set_profiling(False)
+    applies = sum(i["operator_applies"] for i in infos)
+    apply_ms = sum(i["operator_apply_ms"] for i in infos) / max(applies, 1)
+    ncomp = 2  # the meander's filaments run along x and y
+    traffic = apply_bytes(st, 2, ncomp)
+    achieved = traffic / (apply_ms * 1e-3) / 1e9 if apply_ms > 0 else 0.0
+    out = {
+        "metric": "matrix-free freq-point solves/s @99k filaments (pFFT + GMRES)",
+        "value": len(freqs) * steps / elapsed,
+        "unit": "freq-point solves/s",
+        "ms_per_step": elapsed / steps * 1e3,
+        "dtype": "fp32 FFT, fp64 near field and Arnoldi",
+        "config": {"workload": f"meander(157) 2x2, {counts.filaments} filaments, "
+                               f"{counts.internal_meshes} internal loops, 8 log freqs 1 MHz-1 GHz",
+                   "assembly_s": assembly_s, "operator_build_s": infos[0]["inductance_s"],
+                   "grid_dims": st["grid_dims"], "fft_dims": st["fft_dims"],
+                   "projection_entries": st["projection_entries"],
+                   "near_entries": st["near_entries"], "gmres_restart": 30,
+                   "gmres_tolerance": 1e-8, "gmres_iterations": infos[-1]["gmres_iterations"],
+                   "gmres_iterations_max": infos[-1]["gmres_iterations_max"],
+                   "line_group": st["line_group"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "pFFT operator product on the real and imaginary branch "
+                               "currents (spread, 5 pruned FFT passes, gather, near field)",
+                     "bytes_per_apply": traffic, "avg_apply_ms": apply_ms, "applies": applies},
+    }
+    if dense_check is not None:
+        packed49, ports49 = dense_check
+        it49 = IterativeSystem.assemble_packed(packed49, ports49, params)
+        z49, _ = it49.sweep_raw(freqs)
+        it49.close()
+        dense = MeshSystem.assemble_packed(packed49, ports49, "fp64")
+        zd, _ = dense.sweep_raw(freqs)
+        dense.close()
+        out["z_rel_err_vs_dense_9800"] = max(
+            float(np.linalg.norm(z49[k] - zd[k]) / np.linalg.norm(zd[k])) for k in range(len(freqs)))
+    it.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +170,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 sweep the refined result is compared with")
+    ap.add_argument("--workload", default="dense", choices=["dense", "matrix-free"],
+                    help="dense: the headline line (with the matrix-free sweep attached); "
+                         "matrix-free: only the matrix-free sweep, as its own line")
+    ap.add_argument("--no-matrix-free", action="store_true",
+                    help="leave the matrix-free sweep out of the headline line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
@@ -76,6 +182,11 @@ def main() -> int:
         return 2
 
     from fasterhenry_amd import MeshSystem, fixtures, model, native
+
+    if args.workload == "matrix-free":
+        out = matrix_free(args.steps, args.warmup)
+        print(json.dumps(out))
+        return 0
 
     g, ports, d = fixtures.meander_problem(args.bars)
     packed = model.pack(g, ports, d)
@@ -172,6 +283,9 @@ def main() -> int:
         "cpu_baseline": cpu,
         **extra,
     }
+    if not args.no_matrix_free:
+        system.close()
+        out["matrix_free"] = matrix_free(1, 1, dense_check=(packed, ports))
     print(json.dumps(out))
     system.close()
     return 0
