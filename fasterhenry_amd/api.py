@@ -394,6 +394,30 @@ def solve(geometry: Geometry, ports: Sequence[Port], discretization: Discretizat
         system.close()
 
 
+def solve_batch(problems, frequencies, solver: SolverChoice = SolverChoice.DENSE,
+                precision: str = "fp32_refined", concurrency: int = 0) -> tuple:
+    """solve() for every (geometry, ports, discretization) of ``problems``
+    over the same frequencies, scheduled together on the device
+    (fh_solve_batch).  Returns (list of (K, p, p) impedance arrays, list of
+    per-problem sweep-info dicts)."""
+    packed = [pack(g, ports, d) for (g, ports, d) in problems]
+    structs = (native.Problem * len(packed))(*[native.problem_struct(pk) for pk in packed])
+    f = np.ascontiguousarray(frequencies, dtype=np.float64)
+    K = len(f)
+    sizes = [len(pk.ports) for pk in packed]
+    z = np.zeros(sum(K * p * p for p in sizes), dtype=np.complex128)
+    infos = (native.SweepInfo * len(packed))()
+    failed = C.c_size_t()
+    native.check(native.lib().fh_solve_batch(structs, len(packed), solver.value, _precision_code(precision),
+                                             native.ptr(f), K, native.ptr(z), infos, int(concurrency),
+                                             C.byref(failed)))
+    out, o = [], 0
+    for p in sizes:
+        out.append(z[o:o + K * p * p].reshape(K, p, p))
+        o += K * p * p
+    return out, [i.as_dict() for i in infos]
+
+
 def discretize_geometry(geometry: Geometry, discretization: Discretization) -> np.ndarray:
     """All filaments, segment by segment (solve.rs:947-975), from the native host code."""
     packed = pack(geometry, [], discretization)

@@ -8,10 +8,13 @@
 // fit a device-memory budget, and the first failure in list order is
 // reported.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <numeric>
+#include <thread>
 
 #include "fh_internal.hpp"
 #include "fh_iterative.hpp"
@@ -578,6 +581,71 @@ fh_status fh_solve(const fh_problem *problem, fh_solver_choice choice, fh_precis
   st = fh_sweep(s, freqs, K, z_out, info);
   fh_system_destroy(s);
   return st;
+}
+
+fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_choice choice,
+                         fh_precision precision, const double *freqs, size_t K, double *z_out,
+                         fh_sweep_info *infos, int32_t concurrency, size_t *failed) {
+  clear_error();
+  if (failed) *failed = 0;
+  if (count == 0) return FH_OK;
+  if (!problems || (K && (!freqs || !z_out))) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  for (size_t k = 0; k < K; ++k)
+    if (!(std::isfinite(freqs[k]) && freqs[k] >= 0.0))
+      return set_error(FH_E_INVALID_FREQUENCY,
+                       "frequency " + std::to_string(k) + " is " + std::to_string(freqs[k]) +
+                           " Hz; frequencies must be finite and non-negative",
+                       static_cast<int64_t>(k), 0, freqs[k]);
+  fh_status st = check_device();
+  if (st) return st;
+  std::vector<size_t> off(count + 1, 0), work(count, 0);
+  for (size_t b = 0; b < count; ++b) {
+    const size_t p = problems[b].n_ports;
+    off[b + 1] = off[b] + 2 * K * p * p;
+    for (size_t q = 0; q < problems[b].n_segments && problems[b].grids; ++q)
+      work[b] += static_cast<size_t>(problems[b].grids[q].nw) * problems[b].grids[q].nh;
+  }
+  // longest first: the schedule's tail is a small problem
+  std::vector<size_t> order(count);
+  std::iota(order.begin(), order.end(), size_t{0});
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+  const size_t workers = std::min<size_t>(count, concurrency > 0 ? static_cast<size_t>(concurrency) : 4);
+  int device = 0;
+  FH_HIP_TRY(hipGetDevice(&device));
+  struct Outcome {
+    fh_status status = FH_OK;
+    ErrorState error;
+  };
+  std::vector<Outcome> outcome(count);
+  std::atomic<size_t> next{0};
+  auto worker = [&]() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      const size_t q = next.fetch_add(1);
+      if (q >= count) break;
+      const size_t b = order[q];
+      fh_system *sys = nullptr;
+      fh_status s2 = fh_system_create(&problems[b], choice, precision, &sys);
+      if (s2 == FH_OK) {
+        s2 = fh_sweep(sys, freqs, K, z_out + off[b], infos ? &infos[b] : nullptr);
+        fh_system_destroy(sys);
+      }
+      outcome[b].status = s2;
+      if (s2) outcome[b].error = error_state();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t w = 1; w < workers; ++w) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
+  for (size_t b = 0; b < count; ++b)
+    if (outcome[b].status) {
+      error_state() = outcome[b].error;
+      error_state().message = "problem " + std::to_string(b) + ": " + error_state().message;
+      if (failed) *failed = b;
+      return outcome[b].status;
+    }
+  return FH_OK;
 }
 
 fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *L_out,

@@ -9,6 +9,8 @@
 * ``spiral_system`` -- fasterhenry/tests/common/mod.rs:102-157, the 2-turn
   square spiral whose published result is L(1 kHz) = 0.637460 nH and
   R(DC) = 2.844828 ohm (docs/validation.md:107-112).
+* ``batch_problems`` -- the batched-sweep workload: seeded square spirals and
+  busbar pairs of uneven sizes (300 - 4 000 filaments), surface-graded.
 """
 from __future__ import annotations
 
@@ -217,3 +219,76 @@ def random_cloud_problem(n: int, manhattan: bool, seed: int, side: float = None)
         nb = g.add_node(Node(*b))
         g.add_segment(SegmentDef(na, nb, w, h, COPPER))
     return g, [Port(0, 1)], Discretization.uniform(1, 1)
+
+
+def graded_spiral(turns: int, width: float, spacing: float, thickness: float, nw: int, nh: int,
+                  split: int = 1, ratio: float = 2.0):
+    """A square spiral of ``turns`` turns in the plane z = thickness / 2,
+    innermost side 4 (width + spacing) long, pitch width + spacing, each side
+    cut into ``split`` equal segments; one port across its ends; every
+    segment cut nw x nh, graded toward the surfaces by ``ratio``."""
+    pitch = width + spacing
+    sides = square_spiral_segments(turns, 4.0 * pitch, pitch)
+    nodes = [Node(0.0, 0.0, 0.5 * thickness)]
+    for (start, end) in sides:
+        for q in range(1, split + 1):
+            t = q / split
+            nodes.append(Node(start[0] + t * (end[0] - start[0]), start[1] + t * (end[1] - start[1]),
+                              0.5 * thickness))
+    defs = [SegmentDef(i, i + 1, width, thickness, COPPER) for i in range(len(nodes) - 1)]
+    g = Geometry.from_parts(nodes, defs)
+    return g, [Port(0, len(nodes) - 1, "spiral")], Discretization.graded(nw, nh, ratio)
+
+
+def graded_busbar_pair(width: float, spacing: float, thickness: float, nw: int, nh: int,
+                       segments_per_bar: int = 8, length: float = 5e-3, ratio: float = 2.0):
+    """Two parallel bars ``length`` long, ``spacing`` apart edge to edge,
+    shorted at the far end by a strap of the same section; one port across the
+    near ends; every segment cut nw x nh, graded toward the surfaces."""
+    g = Geometry()
+    ya, yb = 0.0, width + spacing
+    z = 0.5 * thickness
+    a = [g.add_node(Node(length * k / segments_per_bar, ya, z)) for k in range(segments_per_bar + 1)]
+    b = [g.add_node(Node(length * k / segments_per_bar, yb, z)) for k in range(segments_per_bar + 1)]
+    for k in range(segments_per_bar):
+        g.add_segment(SegmentDef(a[k], a[k + 1], width, thickness, COPPER))
+    for k in range(segments_per_bar):
+        g.add_segment(SegmentDef(b[k + 1], b[k], width, thickness, COPPER))
+    g.add_segment(SegmentDef(a[-1], b[-1], width, thickness, COPPER))
+    return g, [Port(a[0], b[0])], Discretization.graded(nw, nh, ratio)
+
+
+def _cross_section(width: float, thickness: float):
+    """Surface-graded cells about as wide as tall: 2 or 3 across the
+    thickness, 2 - 10 across the width."""
+    nh = 2 if thickness < 40e-6 else 3
+    nw = max(2, min(10, int(round(nh * width / thickness))))
+    return nw, nh
+
+
+def batch_problems(count: int = 48, seed: int = 48):
+    """The batched-sweep workload: ``count`` geometries from the LCG of
+    fasterhenry/src/pfft/cloud_draws.rs:36-44.  Five draws per geometry, in
+    this order: a turn draw r, width U(50, 500) um, spacing U(50, 300) um,
+    thickness U(17, 70) um, a size draw u.  Even index: a square spiral of
+    2 + floor(7 r) turns; odd index: a busbar pair.  Each segment is cut
+    nw x nh (graded, ratio 2; see _cross_section), and the sides are split
+    into as many segments as bring the filament count closest to
+    300 + 3 700 u."""
+    r = lcg(seed)
+    out = []
+    for k in range(count):
+        r_turn = r()
+        width = 50e-6 + 450e-6 * r()
+        spacing = 50e-6 + 250e-6 * r()
+        thickness = 17e-6 + 53e-6 * r()
+        target = 300 + 3700 * r()
+        nw, nh = _cross_section(width, thickness)
+        if k % 2 == 0:
+            turns = 2 + min(int(7 * r_turn), 6)
+            split = max(1, int(round(target / (4 * turns * nw * nh))))
+            out.append(graded_spiral(turns, width, spacing, thickness, nw, nh, split))
+        else:
+            per_bar = max(1, int(round((target / (nw * nh) - 1) / 2)))
+            out.append(graded_busbar_pair(width, spacing, thickness, nw, nh, per_bar))
+    return out

@@ -29,7 +29,7 @@ SYMBOLS = [
     "fh_partial_inductance", "fh_discretize", "fh_device_math", "fh_last_error",
     "fh_error_detail", "fh_version", "fh_iterative_default_params", "fh_system_create_iterative",
     "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
-    "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense",
+    "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
 ]
 
 _ERRORS = {
@@ -167,6 +167,9 @@ def lib() -> C.CDLL:
         L.fh_pfft_data.argtypes = [vp] * 7
         L.fh_pfft_destroy.restype = None
         L.fh_pfft_destroy.argtypes = [vp]
+        L.fh_solve_batch.restype = st
+        L.fh_solve_batch.argtypes = [vp, sz, C.c_int, C.c_int, vp, sz, vp, vp, C.c_int32,
+                                     C.POINTER(sz)]
         L.fh_gmres_dense.restype = st
         L.fh_gmres_dense.argtypes = [sz, vp, vp, vp, C.POINTER(GmresParams), C.c_int32,
                                      C.POINTER(C.c_uint64), dp, C.POINTER(C.c_int32)]

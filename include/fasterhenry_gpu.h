@@ -246,6 +246,19 @@ void fh_system_destroy(fh_system *system);
 fh_status fh_solve(const fh_problem *problem, fh_solver_choice choice, fh_precision precision,
                    const double *freqs_hz, size_t K, double *z_out, fh_sweep_info *info);
 
+/* A batch of independent problems swept over the same K frequencies -- the
+ * work of one solve() per problem (solve.rs:931-938), scheduled together:
+ * systems are assembled and swept concurrently on the device by
+ * `concurrency` workers (0: the library's choice), largest problems first,
+ * each on its own stream.  Each problem's result is identical to its own
+ * fh_solve.  z_out: problem b's K x p_b x p_b {re, im} values start at
+ * 2 K sum_{a<b} p_a^2 doubles.  infos (may be NULL): one fh_sweep_info per
+ * problem.  On failure the first failing problem in batch order is reported
+ * (its status and detail) and *failed (may be NULL) receives its index. */
+fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_choice choice,
+                         fh_precision precision, const double *freqs_hz, size_t K, double *z_out,
+                         fh_sweep_info *infos, int32_t concurrency, size_t *failed);
+
 /* Kernel level.  partial_inductance_matrix (inductance/batch.rs:246-253):
  * L_out (n x n, row-major, host) for host filaments.  methods (may be NULL)
  * receives the Method of inductance/mod.rs:130-152 per entry (0 orthogonal,
