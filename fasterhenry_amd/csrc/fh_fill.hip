@@ -24,6 +24,7 @@
 // parallel, then added sequentially), so the values are bit-identical to a
 // sequential FP64 evaluation with the same log/atan (fh_math.hpp).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <mutex>
@@ -911,7 +912,8 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
                                                            int8_t *__restrict__ methods,
                                                            const uint2 *__restrict__ queue,
                                                            const unsigned int *__restrict__ queue_count,
-                                                           unsigned long long *__restrict__ err) {
+                                                           unsigned long long *__restrict__ err,
+                                                           int skip_aligned) {
   __shared__ double lds[kCloudCapacity];
   __shared__ Frame stack[kStackCapacity];
   const unsigned int count = *queue_count;
@@ -919,6 +921,7 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
   for (unsigned int e = blockIdx.x; e < count; e += gridDim.x) {
     const uint2 q = queue[e];
     const uint32_t kind = q.x >> 30;
+    if (skip_aligned && kind == kAligned) continue;
     const int i = static_cast<int>(q.x & 0x3fffffffu), j = static_cast<int>(q.y);
     double v;
     int8_t method;
@@ -971,7 +974,8 @@ __global__ void __launch_bounds__(256) pairs_kernel(const fh_filament *__restric
 __global__ void __launch_bounds__(64) pairs_deferred_kernel(
     const fh_filament *__restrict__ F, int n, const uint2 *__restrict__ pairs,
     double *__restrict__ out, const uint2 *__restrict__ queue,
-    const unsigned int *__restrict__ queue_count, unsigned long long *__restrict__ err) {
+    const unsigned int *__restrict__ queue_count, unsigned long long *__restrict__ err,
+    int skip_aligned) {
   __shared__ double lds[kCloudCapacity];
   __shared__ Frame stack[kStackCapacity];
   const unsigned int count = *queue_count;
@@ -979,6 +983,7 @@ __global__ void __launch_bounds__(64) pairs_deferred_kernel(
   for (unsigned int e = blockIdx.x; e < count; e += gridDim.x) {
     const uint2 q = queue[e];
     const uint32_t kind = q.x >> 30;
+    if (skip_aligned && kind == kAligned) continue;
     const size_t k = q.x & 0x3fffffffu;
     const int i = static_cast<int>(pairs[k].x), j = static_cast<int>(pairs[k].y);
     double v;
@@ -997,6 +1002,158 @@ __global__ void __launch_bounds__(64) pairs_deferred_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// The aligned-bar memo (inductance/memo.rs:106-131) as a deduplication of the
+// deferred queue: an aligned pair's integral depends only on its nine
+// AlignedBars numbers, so pairs with the same bits share one evaluation.
+// Keys are hashed, sorted, split into runs of bit-equal keys; one wavefront
+// evaluates each run head; every member takes the head's integral and applies
+// its own scale.  A shared value is the very bits its own evaluation would
+// give.
+
+// Queue entry -> (i, j): tile queues hold (i | kind << 30, j), pair-list
+// queues (k | kind << 30, 0) with (i, j) = pairs[k].
+__device__ inline void entry_pair(uint2 q, const uint2 *pairs, int &i, int &j, uint32_t &kind, uint32_t &k) {
+  kind = q.x >> 30;
+  k = q.x & 0x3fffffffu;
+  if (pairs) {
+    const uint2 pr = pairs[k];
+    i = static_cast<int>(pr.x);
+    j = static_cast<int>(pr.y);
+  } else {
+    i = static_cast<int>(k);
+    j = static_cast<int>(q.y);
+  }
+}
+
+// The AlignedBars of a pair as plan_pair forms them (canonical order).
+__device__ inline void aligned_bars_of(const fh_filament &fa_in, const fh_filament &fb_in, Aligned &ab) {
+  const fh_filament *fa = &fa_in, *fb = &fb_in;
+  if (canonical(*fa, *fb) > 0) {
+    const fh_filament *t = fa;
+    fa = fb;
+    fb = t;
+  }
+  const Bar a = make_bar(*fa), b = make_bar(*fb);
+  (void)aligned_of(a, b, ab);
+}
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+__global__ void memo_key_kernel(const fh_filament *__restrict__ F, const uint2 *__restrict__ queue,
+                                const uint2 *__restrict__ pairs, unsigned count, double *__restrict__ keys,
+                                unsigned long long *__restrict__ hash, unsigned *__restrict__ idx) {
+  const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  idx[e] = e;
+  int i, j;
+  uint32_t kind, k;
+  entry_pair(queue[e], pairs, i, j, kind, k);
+  unsigned long long h = kNoKey;
+  if (kind == kAligned) {
+    Aligned ab;
+    aligned_bars_of(F[i], F[j], ab);
+    const double w[9] = {ab.length[0], ab.length[1], ab.width[0], ab.width[1], ab.height[0],
+                         ab.height[1], ab.offset[0], ab.offset[1], ab.offset[2]};
+    h = 0x9E3779B97F4A7C15ull;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      keys[static_cast<size_t>(e) * 9 + t] = w[t];
+      h ^= static_cast<unsigned long long>(__double_as_longlong(w[t]));
+      h ^= h >> 30;
+      h *= 0xbf58476d1ce4e5b9ull;
+      h ^= h >> 27;
+      h *= 0x94d049bb133111ebull;
+      h ^= h >> 31;
+    }
+    if (h == kNoKey) h = kNoKey - 1;
+  }
+  hash[e] = h;
+}
+
+__global__ void memo_head_kernel(const unsigned long long *__restrict__ sh, const unsigned *__restrict__ si,
+                                 const double *__restrict__ keys, unsigned count, unsigned *__restrict__ head) {
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  unsigned hd = 0;
+  if (sh[q] != kNoKey) {
+    hd = 1;
+    if (q > 0 && sh[q - 1] == sh[q]) {
+      bool same = true;
+      for (int t = 0; t < 9; ++t)
+        same = same && __double_as_longlong(keys[static_cast<size_t>(si[q]) * 9 + t]) ==
+                           __double_as_longlong(keys[static_cast<size_t>(si[q - 1]) * 9 + t]);
+      if (same) hd = 0;
+    }
+  }
+  head[q] = hd;
+}
+
+__global__ void memo_compact_kernel(const unsigned *__restrict__ head, const unsigned *__restrict__ run,
+                                    const unsigned *__restrict__ si, unsigned count, unsigned *__restrict__ uq,
+                                    unsigned *__restrict__ ucount) {
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  if (head[q]) uq[run[q] - 1] = si[q];
+  if (q == count - 1) *ucount = run[q];
+}
+
+__global__ void __launch_bounds__(64) memo_eval_kernel(const fh_filament *__restrict__ F,
+                                                       const uint2 *__restrict__ queue,
+                                                       const uint2 *__restrict__ pairs,
+                                                       const unsigned *__restrict__ uq,
+                                                       const unsigned *__restrict__ ucount,
+                                                       double *__restrict__ uin, int8_t *__restrict__ umethod) {
+  __shared__ double lds[kCloudCapacity];
+  __shared__ Frame stack[kStackCapacity];
+  const unsigned count = *ucount;
+  for (unsigned u = blockIdx.x; u < count; u += gridDim.x) {
+    int i, j;
+    uint32_t kind, k;
+    entry_pair(queue[uq[u]], pairs, i, j, kind, k);
+    Aligned ab;
+    aligned_bars_of(F[i], F[j], ab);
+    int8_t method;
+    const double in = aligned_dispatch_wave(ab, method, stack, lds);
+    if (threadIdx.x == 0) {
+      uin[u] = in;
+      umethod[u] = method;
+    }
+    __syncthreads();
+  }
+}
+
+// Every aligned entry: v = 1e-7 (integral * sign(cos) / (area_a area_b)), as
+// deferred_eval scales it, into L (both triangles) or the pair list.
+__global__ void memo_scatter_kernel(const fh_filament *__restrict__ F, int n, const uint2 *__restrict__ queue,
+                                    const uint2 *__restrict__ pairs, const unsigned long long *__restrict__ sh,
+                                    const unsigned *__restrict__ si, const unsigned *__restrict__ run,
+                                    unsigned count, const double *__restrict__ uin,
+                                    const int8_t *__restrict__ umethod, double *__restrict__ L,
+                                    int8_t *__restrict__ methods, double *__restrict__ out,
+                                    unsigned long long *__restrict__ err) {
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count || sh[q] == kNoKey) return;
+  int i, j;
+  uint32_t kind, k;
+  entry_pair(queue[si[q]], pairs, i, j, kind, k);
+  const fh_filament &fa = F[i], &fb = F[j];
+  const double cos_ = dot(ld3(fa.dir), ld3(fb.dir));
+  const double scale = signum(cos_) / ((fa.width * fa.height) * (fb.width * fb.height));
+  const double v = 1.0e-7 * (uin[run[q] - 1] * scale);
+  if (!__builtin_isfinite(v)) record_error(err, i, j, n, 2);
+  if (pairs) {
+    out[k] = v;
+  } else {
+    L[static_cast<size_t>(i) * n + j] = v;
+    L[static_cast<size_t>(j) * n + i] = v;
+    if (methods) {
+      methods[static_cast<size_t>(i) * n + j] = umethod[run[q] - 1];
+      methods[static_cast<size_t>(j) * n + i] = umethod[run[q] - 1];
+    }
+  }
+}
+
 __global__ void device_math_kernel(const double *x, size_t n, double *lg, double *at) {
   const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
   if (k < n) {
@@ -1008,6 +1165,65 @@ __global__ void device_math_kernel(const double *x, size_t n, double *lg, double
 }  // namespace
 
 fh_status fill_tables_ready() { return upload_tables(); }
+
+namespace {
+
+constexpr unsigned kMemoMin = 256;  // shorter queues are evaluated pair by pair
+
+// Runs the memo over the aligned entries of a deferred queue (count read from
+// the device); `used` = false when the queue is too short to bother, and the
+// plain deferred kernel must then evaluate the aligned entries too.
+fh_status memo_aligned(const fh_filament *d_F, int n, const uint2 *queue, const uint2 *pairs,
+                       const unsigned int *d_count, double *d_L, int8_t *d_methods, double *d_out,
+                       unsigned long long *err, hipStream_t stream, bool &used) {
+  used = false;
+  unsigned count = 0;
+  FH_HIP_TRY(hipMemcpyAsync(&count, d_count, sizeof(count), hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));
+  if (count < kMemoMin) return FH_OK;
+  fh_status st;
+  DeviceArray<double> keys, uin;
+  DeviceArray<unsigned long long> hash, shash;
+  DeviceArray<unsigned> idx, sidx, head, run, uq, ucount;
+  DeviceArray<int8_t> umethod;
+  DeviceArray<char> temp;
+  if ((st = keys.allocate(static_cast<size_t>(count) * 9)) || (st = hash.allocate(count)) ||
+      (st = shash.allocate(count)) || (st = idx.allocate(count)) || (st = sidx.allocate(count)) ||
+      (st = head.allocate(count)) || (st = run.allocate(count)) || (st = uq.allocate(count)) ||
+      (st = ucount.allocate(1)) || (st = uin.allocate(count)) || (st = umethod.allocate(count)))
+    return st;
+  const unsigned blocks = (count + 255) / 256;
+  memo_key_kernel<<<blocks, 256, 0, stream>>>(d_F, queue, pairs, count, keys.ptr, hash.ptr, idx.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  size_t sort_bytes = 0, scan_bytes = 0;
+  FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, hash.ptr, shash.ptr, idx.ptr, sidx.ptr,
+                                               static_cast<int>(count), 0, 64, stream));
+  FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_bytes, head.ptr, run.ptr, static_cast<int>(count),
+                                              stream));
+  if ((st = temp.allocate(std::max(sort_bytes, scan_bytes)))) return st;
+  FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(temp.ptr, sort_bytes, hash.ptr, shash.ptr, idx.ptr, sidx.ptr,
+                                               static_cast<int>(count), 0, 64, stream));
+  memo_head_kernel<<<blocks, 256, 0, stream>>>(shash.ptr, sidx.ptr, keys.ptr, count, head.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  size_t sb = std::max(sort_bytes, scan_bytes);
+  FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(temp.ptr, sb, head.ptr, run.ptr, static_cast<int>(count), stream));
+  memo_compact_kernel<<<blocks, 256, 0, stream>>>(head.ptr, run.ptr, sidx.ptr, count, uq.ptr, ucount.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  int dev = 0, cus = 256;
+  FH_HIP_TRY(hipGetDevice(&dev));
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  memo_eval_kernel<<<static_cast<unsigned>(cus) * 16, 64, 0, stream>>>(d_F, queue, pairs, uq.ptr, ucount.ptr,
+                                                                       uin.ptr, umethod.ptr);
+  FH_HIP_TRY(hipGetLastError());
+  memo_scatter_kernel<<<blocks, 256, 0, stream>>>(d_F, n, queue, pairs, shash.ptr, sidx.ptr, run.ptr, count,
+                                                  uin.ptr, umethod.ptr, d_L, d_methods, d_out, err);
+  FH_HIP_TRY(hipGetLastError());
+  FH_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
+  used = true;
+  return FH_OK;
+}
+
+}  // namespace
 
 fh_status fill_inductance(const fh_filament *d_F, size_t n, double *d_L, int8_t *d_methods,
                           hipStream_t stream) {
@@ -1033,8 +1249,12 @@ fh_status fill_inductance(const fh_filament *d_F, size_t n, double *d_L, int8_t 
   int dev = 0, cus = 256;
   FH_HIP_TRY(hipGetDevice(&dev));
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  bool memo = false;
+  if ((st = memo_aligned(d_F, static_cast<int>(n), queue.ptr, nullptr, qcount.ptr, d_L, d_methods, nullptr,
+                         err.ptr, stream, memo)))
+    return st;
   fill_deferred_kernel<<<static_cast<unsigned>(cus) * 16, 64, 0, stream>>>(
-      d_F, static_cast<int>(n), d_L, d_methods, queue.ptr, qcount.ptr, err.ptr);
+      d_F, static_cast<int>(n), d_L, d_methods, queue.ptr, qcount.ptr, err.ptr, memo ? 1 : 0);
   FH_HIP_TRY(hipGetLastError());
   unsigned long long key = 0;
   FH_HIP_TRY(hipMemcpyAsync(&key, err.ptr, sizeof(key), hipMemcpyDeviceToHost, stream));
@@ -1075,8 +1295,12 @@ fh_status fill_pairs(const fh_filament *d_F, size_t n, const uint2 *d_pairs, siz
   int dev = 0, cus = 256;
   FH_HIP_TRY(hipGetDevice(&dev));
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  bool memo = false;
+  if ((st = memo_aligned(d_F, static_cast<int>(n), queue.ptr, d_pairs, qcount.ptr, nullptr, nullptr, d_out,
+                         err.ptr, stream, memo)))
+    return st;
   pairs_deferred_kernel<<<static_cast<unsigned>(cus) * 16, 64, 0, stream>>>(
-      d_F, static_cast<int>(n), d_pairs, d_out, queue.ptr, qcount.ptr, err.ptr);
+      d_F, static_cast<int>(n), d_pairs, d_out, queue.ptr, qcount.ptr, err.ptr, memo ? 1 : 0);
   FH_HIP_TRY(hipGetLastError());
   unsigned long long key = 0;
   FH_HIP_TRY(hipMemcpyAsync(&key, err.ptr, sizeof(key), hipMemcpyDeviceToHost, stream));

@@ -33,6 +33,16 @@ void clear_error();
                                            hipGetErrorString(e_));                      \
   } while (0)
 
+// Device memory.  Normally hipMalloc / hipFree.  A thread may turn on a
+// block cache (the batch workers do, between problems): freed blocks then
+// stay with the thread and are handed out again to its later allocations of
+// up to the same size, which avoids hipFree's device-wide synchronization.
+// Every entry point synchronizes its stream before returning, so a block is
+// idle when it reaches the cache; turning the cache off frees its blocks.
+void *device_alloc(size_t bytes);  // nullptr on failure
+void device_free(void *p);
+void block_cache(bool on);
+
 // Device buffer owned by a system; freed in its destructor.
 template <typename T>
 struct DeviceArray {
@@ -43,19 +53,17 @@ struct DeviceArray {
   DeviceArray &operator=(const DeviceArray &) = delete;
   ~DeviceArray() { release(); }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) device_free(ptr);
     ptr = nullptr;
     count = 0;
   }
   fh_status allocate(size_t n) {
     release();
     if (n == 0) return FH_OK;
-    if (hipMalloc(&ptr, n * sizeof(T)) != hipSuccess) {
-      ptr = nullptr;
-      (void)hipGetLastError();
+    ptr = static_cast<T *>(device_alloc(n * sizeof(T)));
+    if (!ptr)
       return set_error(FH_E_NO_MEMORY, "device allocation of " + std::to_string(n * sizeof(T)) +
                                            " bytes failed");
-    }
     count = n;
     return FH_OK;
   }

@@ -12,9 +12,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 #include "fh_internal.hpp"
 #include "fh_iterative.hpp"
@@ -43,12 +45,82 @@ fh_status set_error(fh_status status, const std::string &message, int64_t index0
 
 void clear_error() { error_state() = ErrorState{}; }
 
+namespace {
+
+struct BlockCache {
+  bool on = false;
+  std::multimap<size_t, void *> idle;             // size -> block
+  std::unordered_map<void *, size_t> size_of;     // every block this thread's cache handed out
+  ~BlockCache() { flush(); }
+  void flush() {
+    for (auto &kv : idle) (void)hipFree(kv.second);
+    idle.clear();
+    // blocks still in use go back to plain hipFree when released
+    size_of.clear();
+  }
+};
+
+BlockCache &cache() {
+  thread_local BlockCache c;
+  return c;
+}
+
+}  // namespace
+
+void *device_alloc(size_t bytes) {
+  BlockCache &c = cache();
+  if (c.on) {
+    auto it = c.idle.lower_bound(bytes);
+    if (it != c.idle.end() && it->first <= 2 * bytes + (size_t{1} << 20)) {
+      void *p = it->second;
+      c.idle.erase(it);
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    if (!c.on || c.idle.empty()) return nullptr;
+    // room may be held by idle blocks
+    for (auto &kv : c.idle) {
+      c.size_of.erase(kv.second);
+      (void)hipFree(kv.second);
+    }
+    c.idle.clear();
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  if (c.on) c.size_of[p] = bytes;
+  return p;
+}
+
+void device_free(void *p) {
+  BlockCache &c = cache();
+  if (c.on) {
+    auto it = c.size_of.find(p);
+    if (it != c.size_of.end()) {
+      c.idle.emplace(it->second, p);
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void block_cache(bool on) {
+  BlockCache &c = cache();
+  if (!on) c.flush();
+  c.on = on;
+}
+
 }  // namespace fh
 
 using namespace fh;
 
 struct fh_system {
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   fh_solver solver = FH_SOLVER_DENSE;
   std::unique_ptr<IterativeSystem> iter;
   fh_precision precision = FH_FP64_PLAIN;
@@ -76,7 +148,7 @@ struct fh_system {
     d_L.release();
     d_lmesh.release();
     ws.release();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -92,6 +164,19 @@ struct fh_pfft {
 };
 
 namespace {
+
+// A batch worker's stream, lent to the systems it creates (none elsewhere).
+thread_local hipStream_t tl_worker_stream = nullptr;
+
+fh_status system_stream(fh_system &s) {
+  if (tl_worker_stream) {
+    s.stream = tl_worker_stream;
+    s.own_stream = false;
+    return FH_OK;
+  }
+  FH_HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+  return FH_OK;
+}
 
 double seconds_since(Clock::time_point t0) {
   return std::chrono::duration<double>(Clock::now() - t0).count();
@@ -189,7 +274,7 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
     const fh_filament &f = s->filaments[i];
     s->resistances[i] = f.length / (f.sigma * (f.width * f.height));  // solve.rs:941-943
   }
-  FH_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if ((st = system_stream(*s))) return st;
   if ((st = s->d_filaments.allocate(n))) return st;
   if ((st = s->d_L.allocate(n * n))) return st;
   if (n) {
@@ -309,7 +394,7 @@ fh_status fh_system_create_iterative(const fh_problem *problem, const fh_iterati
     const fh_filament &f = s->filaments[i];
     s->resistances[i] = f.length / (f.sigma * (f.width * f.height));  // solve.rs:941-943
   }
-  FH_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if ((st = system_stream(*s))) return st;
   s->iter.reset(new IterativeSystem());
   if ((st = s->iter->build(s->filaments, s->resistances, s->basis, *params, s->stream))) return st;
   s->inductance_s = s->iter->operator_seconds();
@@ -620,6 +705,13 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
   std::atomic<size_t> next{0};
   auto worker = [&]() {
     (void)hipSetDevice(device);
+    hipStream_t stream = nullptr;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      stream = nullptr;
+    }
+    tl_worker_stream = stream;
+    block_cache(true);
     for (;;) {
       const size_t q = next.fetch_add(1);
       if (q >= count) break;
@@ -633,6 +725,9 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
       outcome[b].status = s2;
       if (s2) outcome[b].error = error_state();
     }
+    block_cache(false);
+    tl_worker_stream = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
   };
   std::vector<std::thread> pool;
   for (size_t w = 1; w < workers; ++w) pool.emplace_back(worker);
