@@ -191,6 +191,24 @@ class MeshSystem:
         native.check(native.lib().fh_system_inductance(self._h, native.ptr(out)))
         return out
 
+    def grad(self, frequencies, d_nodes, d_widths=None, d_heights=None) -> tuple:
+        """Z and its derivatives along n_params geometry directions (fh_grad;
+        the reference has no gradients).  d_nodes: (n_params, n_nodes, 3) node
+        velocities; d_widths / d_heights: (n_params, n_segments) or None.
+        Returns (z (K, p, p), dR_dp (K, n_params, p, p), dL_dp (K, n_params,
+        p, p)) with dR = Re dZ/dp and dL = Im dZ/dp / w (0 at f = 0)."""
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        K, p = len(f), self._counts.ports
+        arrays, pmap = _param_map(self._packed, d_nodes, d_widths, d_heights)
+        P = pmap.n_params
+        z = np.zeros((K, p, p), dtype=np.complex128)
+        dr = np.zeros((K, P, p, p))
+        dl = np.zeros((K, P, p, p))
+        native.check(native.lib().fh_grad(self._h, C.byref(pmap), native.ptr(f), K, native.ptr(z),
+                                          native.ptr(dr), native.ptr(dl)))
+        del arrays
+        return z, dr, dl
+
     def blocks(self) -> Dict[str, np.ndarray]:
         m, p = self._counts.internal_meshes, self._counts.ports
         out = [np.zeros((m, m)), np.zeros((m, m)), np.zeros((m, p)), np.zeros((m, p)),
@@ -432,6 +450,67 @@ def discretize_packed(packed: PackedProblem) -> np.ndarray:
     out = np.zeros(n.value, dtype=FILAMENT_DTYPE)
     native.check(L.fh_discretize(C.byref(prob), native.ptr(out), n.value, C.byref(n)))
     return out
+
+
+def _param_map(packed: PackedProblem, d_nodes, d_widths, d_heights) -> tuple:
+    nn, ns = len(packed.nodes), len(packed.segments)
+    dn = np.ascontiguousarray(d_nodes, dtype=np.float64)
+    if dn.ndim == 2:
+        dn = dn[None]
+    P = dn.shape[0]
+    if dn.shape != (P, nn, 3):
+        raise InvalidArgument(f"d_nodes must be (n_params, {nn}, 3), got {dn.shape}")
+    out = [dn]
+    ptrs = []
+    for d in (d_widths, d_heights):
+        if d is None:
+            ptrs.append(None)
+            continue
+        a = np.ascontiguousarray(d, dtype=np.float64).reshape(P, -1)
+        if a.shape != (P, ns):
+            raise InvalidArgument(f"segment derivatives must be (n_params, {ns}), got {a.shape}")
+        out.append(a)
+        ptrs.append(native.ptr(a))
+    pmap = native.ParamMap(P, native.ptr(dn), ptrs[0], ptrs[1])
+    return out, pmap
+
+
+def discretize_tangent(packed: PackedProblem, d_nodes, d_widths=None, d_heights=None) -> np.ndarray:
+    """d(filaments)/dp along one geometry direction (fh_discretize_tangent):
+    d_nodes (n_nodes, 3), d_widths / d_heights (n_segments,) or None.  The
+    filament counts and grading ratios are held at the problem's."""
+    L = native.lib()
+    prob = native.problem_struct(packed)
+    n = C.c_size_t()
+    native.check(L.fh_discretize(C.byref(prob), None, 0, C.byref(n)))
+    arrays, pmap = _param_map(packed, np.asarray(d_nodes, dtype=np.float64)[None],
+                              None if d_widths is None else np.asarray(d_widths)[None],
+                              None if d_heights is None else np.asarray(d_heights)[None])
+    out = np.zeros(n.value, dtype=FILAMENT_DTYPE)
+    native.check(L.fh_discretize_tangent(C.byref(prob), pmap.d_nodes, pmap.d_widths, pmap.d_heights,
+                                         native.ptr(out), n.value))
+    del arrays
+    return out
+
+
+def partial_inductance_derivative(filaments: np.ndarray, tangents: np.ndarray, pairs) -> tuple:
+    """Nominal partial inductances and their derivatives along filament
+    tangents for a list of pairs (fh_partial_inductance_derivative).
+    tangents: (n_dirs, n) filament records; pairs: (count, 3) rows of
+    (f, g, direction).  Returns (values, derivatives)."""
+    f = np.ascontiguousarray(filaments, dtype=FILAMENT_DTYPE)
+    t = np.ascontiguousarray(tangents, dtype=FILAMENT_DTYPE)
+    if t.ndim == 1:
+        t = t[None]
+    if t.shape[1] != len(f):
+        raise InvalidArgument("tangents must be (n_dirs, n_filaments)")
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 3)
+    v = np.zeros(len(pr))
+    d = np.zeros(len(pr))
+    native.check(native.lib().fh_partial_inductance_derivative(
+        native.ptr(f), len(f), native.ptr(t), t.shape[0], native.ptr(pr), len(pr), native.ptr(v),
+        native.ptr(d)))
+    return v, d
 
 
 def partial_inductance_matrix(filaments: np.ndarray, with_methods: bool = False):

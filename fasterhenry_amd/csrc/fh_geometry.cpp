@@ -247,6 +247,96 @@ fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out)
   return FH_OK;
 }
 
+// The derivative of discretize_problem along one direction of the geometry:
+// node coordinates move by dn (n_nodes x 3), segment widths by dw and
+// heights by dh (n_segments each, may be null).  Counts, grading ratios and
+// the frame rule are held fixed, so every filament quantity is a smooth
+// function of the geometry: out[f] holds d(start), d(end), d(length),
+// d(dir), d(wdir), d(hdir), d(width), d(height) of filament f (sigma 0).
+fh_status discretize_tangent(const fh_problem &p, const double *dn, const double *dw, const double *dh,
+                             std::vector<fh_filament> &out) {
+  fh_status st = validate_geometry(p);
+  if (st) return st;
+  out.clear();
+  std::vector<double> wc, we, hc, he;
+  for (size_t s = 0; s < p.n_segments; ++s) {
+    const fh_segment &seg = p.segments[s];
+    const fh_grid &g = p.grids[s];
+    const V3 a = load(p.nodes + 3 * seg.a), b = load(p.nodes + 3 * seg.b);
+    const V3 da = load(dn + 3 * seg.a), db = load(dn + 3 * seg.b);
+    const double W = seg.width, H = seg.height;
+    const double dW = dw ? dw[s] : 0.0, dH = dh ? dh[s] : 0.0;
+    V3 l, w, h;
+    if (int e = segment_frame(a, b, seg, l, w, h))
+      return set_error(FH_E_DISCRETIZE, "segment " + std::to_string(s) + ": " + kSegmentReasons[e],
+                       static_cast<int64_t>(s), 0, 0.0, e);
+    // l = span / |span|
+    const V3 span = sub(b, a), dspan = sub(db, da);
+    const double len = norm(span);
+    const V3 dl = dvd(sub(dspan, mul(l, dot(l, dspan))), len);
+    // the width hint and its derivative
+    V3 hint, dhint{0.0, 0.0, 0.0};
+    if (seg.has_width_dir) {
+      V3 dir = load(seg.width_dir);
+      dir = dvd(dir, norm(dir));
+      // hint = dir - l (dir . l) with dir fixed
+      hint = sub(dir, mul(l, dot(dir, l)));
+      dhint = sub(mul(dl, -dot(dir, l)), mul(l, dot(dir, dl)));
+    } else {
+      const V3 z{0.0, 0.0, 1.0};
+      const V3 in_plane = cross(z, l);
+      if (norm(in_plane) < 1.0e-6) {
+        hint = V3{1.0, 0.0, 0.0};
+      } else {
+        hint = in_plane;
+        dhint = cross(z, dl);
+      }
+    }
+    // w = wv / |wv|, wv = hint - l (hint . l)
+    const V3 wv = sub(hint, mul(l, dot(hint, l)));
+    const double hl = dot(hint, l), dhl = dot(dhint, l) + dot(hint, dl);
+    const V3 dwv = sub(sub(dhint, mul(dl, hl)), mul(l, dhl));
+    const double nwv = norm(wv);
+    const V3 dwdir = dvd(sub(dwv, mul(w, dot(w, dwv))), nwv);
+    const V3 dhdir = add(cross(dl, w), cross(l, dwdir));
+    const double dlen = dot(l, dspan);
+    auto emit = [&](double cw, double dcw, double ch, double dch, double ew, double dew, double eh,
+                    double deh) {
+      const V3 doffset = add(add(mul(dwdir, cw), mul(w, dcw)), add(mul(dhdir, ch), mul(h, dch)));
+      fh_filament t;
+      store(t.start, add(da, doffset));
+      store(t.end, add(db, doffset));
+      t.length = dlen;
+      store(t.dir, dl);
+      store(t.wdir, dwdir);
+      store(t.hdir, dhdir);
+      t.width = dew;
+      t.height = deh;
+      t.sigma = 0.0;
+      out.push_back(t);
+    };
+    const bool graded = g.graded && !(g.width_ratio == 1.0 && g.height_ratio == 1.0);
+    if (graded) {
+      // cells scale with the section: centre and extent are W times fixed fractions
+      graded_cells(W, g.nw, g.width_ratio, wc, we);
+      graded_cells(H, g.nh, g.height_ratio, hc, he);
+      for (uint32_t j = 0; j < g.nh; ++j)
+        for (uint32_t i = 0; i < g.nw; ++i)
+          emit(wc[i], dW * (wc[i] / W), hc[j], dH * (hc[j] / H), we[i], dW * (we[i] / W), he[j],
+               dH * (he[j] / H));
+    } else {
+      for (uint32_t j = 0; j < g.nh; ++j) {
+        const double fj = (static_cast<double>(j) + 0.5) / static_cast<double>(g.nh) - 0.5;
+        for (uint32_t i = 0; i < g.nw; ++i) {
+          const double fi = (static_cast<double>(i) + 0.5) / static_cast<double>(g.nw) - 0.5;
+          emit(fi * W, fi * dW, fj * H, fj * dH, W / g.nw, dW / g.nw, H / g.nh, dH / g.nh);
+        }
+      }
+    }
+  }
+  return FH_OK;
+}
+
 // mesh.rs:170-255 (MeshMatrix::build) and mesh.rs:290-361 (Forest).
 fh_status build_loop_basis(const fh_problem &p, const std::vector<size_t> &per_segment,
                            LoopBasis &out) {

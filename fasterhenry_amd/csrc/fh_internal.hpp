@@ -82,6 +82,10 @@ fh_status validate_geometry(const fh_problem &p);
 fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out);
 fh_status build_loop_basis(const fh_problem &p, const std::vector<size_t> &per_segment,
                            LoopBasis &out);
+// d(filaments) along a direction of the geometry (node moves dn, width and
+// height changes dw, dh; null = 0), counts and frames' rules held fixed.
+fh_status discretize_tangent(const fh_problem &p, const double *dn, const double *dw, const double *dh,
+                             std::vector<fh_filament> &out);
 
 // Pair kernel tables in device memory (fh_fill.hip).
 fh_status fill_tables_ready();

@@ -1458,7 +1458,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     out.refine_steps.assign(K, 0);
     out.refine_change.assign(K, INFINITY);
     out.converged.assign(K, 0);
-    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0);
+    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0), first_dx(K, 0.0);
     for (int it = 1; it <= kMaxRefine; ++it) {
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
       if ((st = factor_solve<float>(L, F, V, DX, P, stream))) return st;
@@ -1498,16 +1498,25 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
           sz += zq * zq;
         }
         const double ndx = std::sqrt(sdx), nr = std::sqrt(sr), nz = std::sqrt(sz);
-        double bound = INFINITY;
+        double bound = INFINITY, x_bound = INFINITY;
+        bool stalled = false;
+        if (it == 1) first_dx[f] = ndx;  // X starts at 0: the first correction is X
         if (it >= 2 && prev_dx[f] > 0.0) {
           const double rho = ndx / prev_dx[f];
-          if (rho <= 0.5 && nz > 0.0) bound = rho / (1.0 - rho) * ndx * nr / nz;
+          if (rho <= 0.5) x_bound = rho / (1.0 - rho) * ndx;
+          if (rho <= 0.5 && nz > 0.0) bound = x_bound * nr / nz;
+          stalled = it >= 3 && rho > 0.5;
         }
         prev_dx[f] = ndx;
         // Converged when the bound on the error of Z is small, when the
         // first-order term is negligible, or when that term has stopped
         // shrinking at a level the corrected Z absorbs.
-        if (bound <= kZTol || worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f])))
+        bool done = bound <= kZTol || worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f]));
+        // x itself to working accuracy when asked for (it enters gradients
+        // to first order): the correction's bound under 1e-14 |x|, or the
+        // contraction stalled at the rounding level, or x exact.
+        if (done && in.tight_x) done = ndx == 0.0 || x_bound <= 1e-14 * first_dx[f] || stalled;
+        if (done)
           out.converged[f] = 1;
         else
           all = false;
@@ -1529,6 +1538,8 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   }
   FH_HIP_TRY(hipMemcpyAsync(out.singular.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost,
                             stream));
+  if (in.d_x_out)
+    FH_HIP_TRY(hipMemcpyAsync(in.d_x_out, X, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   FH_HIP_TRY(hipStreamSynchronize(stream));
   return FH_OK;
 }

@@ -23,6 +23,13 @@ struct DenseSolveInputs {
   const double *d_ree_val = nullptr;
   const double *d_rep = nullptr;
   const double *d_rpp = nullptr;  // r_pp dense p x p
+  // refined form only, may be null: receives x = Zee^-1 Zep of every
+  // frequency, 2 K p vectors of m (vector 2 (k p + q) the real part of
+  // column q at frequency k, the next one its imaginary part)
+  double *d_x_out = nullptr;
+  // refined form: also refine x to working accuracy before stopping (the
+  // default stops when Z, which is stationary in x, has converged)
+  bool tight_x = false;
 };
 
 // Grow-only device workspace.

@@ -18,6 +18,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "fh_adjoint.hpp"
 #include "fh_internal.hpp"
 #include "fh_iterative.hpp"
 #include "fh_ldlt.hpp"
@@ -142,6 +143,28 @@ struct fh_system {
   DeviceArray<double> d_ree_val, d_rep, d_rpp;
   LdltWorkspace ws;
   double inductance_s = 0.0, assembly_s = 0.0;
+  // a copy of the problem (geometry tangents for gradients)
+  std::vector<double> prob_nodes;
+  std::vector<fh_segment> prob_segments;
+  std::vector<fh_grid> prob_grids;
+  std::vector<fh_port> prob_ports;
+  void keep_problem(const fh_problem &p) {
+    prob_nodes.assign(p.nodes, p.nodes + 3 * p.n_nodes);
+    prob_segments.assign(p.segments, p.segments + p.n_segments);
+    prob_grids.assign(p.grids, p.grids + p.n_segments);
+    prob_ports.assign(p.ports, p.ports + p.n_ports);
+  }
+  fh_problem problem() const {
+    fh_problem p;
+    p.nodes = prob_nodes.data();
+    p.n_nodes = prob_nodes.size() / 3;
+    p.segments = prob_segments.data();
+    p.grids = prob_grids.data();
+    p.n_segments = prob_segments.size();
+    p.ports = prob_ports.data();
+    p.n_ports = prob_ports.size();
+    return p;
+  }
   ~fh_system() {
     iter.reset();
     d_filaments.release();
@@ -239,6 +262,19 @@ fh_status fh_discretize(const fh_problem *problem, fh_filament *out, size_t capa
   return FH_OK;
 }
 
+fh_status fh_discretize_tangent(const fh_problem *problem, const double *d_nodes,
+                                const double *d_widths, const double *d_heights, fh_filament *out,
+                                size_t capacity) {
+  clear_error();
+  if (!problem || !d_nodes || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  std::vector<fh_filament> t;
+  fh_status st = discretize_tangent(*problem, d_nodes, d_widths, d_heights, t);
+  if (st) return st;
+  if (capacity < t.size()) return set_error(FH_E_INVALID_ARGUMENT, "capacity too small");
+  std::memcpy(out, t.data(), t.size() * sizeof(fh_filament));
+  return FH_OK;
+}
+
 fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
                            fh_precision precision, fh_system **out) {
   clear_error();
@@ -264,6 +300,7 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
   std::unique_ptr<fh_system> s(new fh_system());
   s->precision = precision;
   if ((st = discretize_problem(*problem, s->filaments))) return st;
+  s->keep_problem(*problem);
   const size_t n = s->filaments.size();
   std::vector<size_t> per(problem->n_segments);
   for (size_t q = 0; q < problem->n_segments; ++q)
@@ -384,6 +421,7 @@ fh_status fh_system_create_iterative(const fh_problem *problem, const fh_iterati
   std::unique_ptr<fh_system> s(new fh_system());
   s->solver = FH_SOLVER_ITERATIVE;
   if ((st = discretize_problem(*problem, s->filaments))) return st;
+  s->keep_problem(*problem);
   const size_t n = s->filaments.size();
   std::vector<size_t> per(problem->n_segments);
   for (size_t q = 0; q < problem->n_segments; ++q)
@@ -839,6 +877,261 @@ fh_status fh_device_math(const double *x, size_t n, double *log_out, double *ata
   fh_status st = check_device();
   if (st) return st;
   return device_math(x, n, log_out, atan_out);
+}
+
+// Gradients (no counterpart in the reference): Z_ab = i_a^T Z_f i_b with
+// Z_f = diag(R) + jw L over filaments and i_a = M^T lambda_a, lambda_a =
+// [-x_a; e_a] the loop currents of port a (x = Zee^-1 Zep).  Z is stationary
+// in lambda, so dZ_ab/dp = i_a^T (dR/dp + jw dL/dp) i_b.
+fh_status fh_grad(fh_system *s, const fh_param_map *map, const double *freqs, size_t K, double *z_out,
+                  double *dR_dp, double *dL_dp) {
+  clear_error();
+  if (!s || !map || (K && !freqs)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  const size_t P = map->n_params;
+  if (K && P && (!dR_dp || !dL_dp)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (s->iter || s->solver != FH_SOLVER_DENSE)
+    return set_error(FH_E_UNSUPPORTED, "gradients need a dense system");
+  for (size_t k = 0; k < K; ++k)
+    if (!(std::isfinite(freqs[k]) && freqs[k] >= 0.0))
+      return set_error(FH_E_INVALID_FREQUENCY,
+                       "frequency " + std::to_string(k) + " is " + std::to_string(freqs[k]) +
+                           " Hz; frequencies must be finite and non-negative",
+                       static_cast<int64_t>(k), 0, freqs[k]);
+  const size_t n = s->filaments.size(), m = s->basis.internal, p = s->basis.loops - m;
+  if (K == 0) return FH_OK;
+  if (p == 0) return set_error(FH_E_INVALID_ARGUMENT, "the problem has no ports");
+  const size_t per_pass = 128 / (p * p);
+  if (per_pass == 0) return set_error(FH_E_UNSUPPORTED, "gradients support at most 11 ports");
+  const fh_problem prob = s->problem();
+  // Filament tangents along every direction (host), then dR_f/dp.
+  std::vector<fh_filament> T(P * n);
+  for (size_t q = 0; q < P; ++q) {
+    std::vector<fh_filament> t;
+    fh_status st = discretize_tangent(
+        prob, map->d_nodes ? map->d_nodes + q * 3 * prob.n_nodes : nullptr,
+        map->d_widths ? map->d_widths + q * prob.n_segments : nullptr,
+        map->d_heights ? map->d_heights + q * prob.n_segments : nullptr, t);
+    if (st) return st;
+    if (t.size() != n) return set_error(FH_E_INVALID_ARGUMENT, "tangent size mismatch");
+    std::copy(t.begin(), t.end(), T.begin() + q * n);
+  }
+  // Loop currents: x = Zee^-1 Zep of every frequency, refined to working
+  // accuracy (they enter the gradient to first order), and Z itself.
+  std::vector<double> z(2 * K * p * p);
+  std::vector<double> X(2 * K * p * m);
+  if (m == 0) {
+    fh_status st = fh_sweep(s, freqs, K, z.data(), nullptr);
+    if (st) return st;
+  } else {
+    const size_t per = ldlt_bytes_per_frequency(m, p, true) + 2 * p * m * sizeof(double);
+    const size_t chunk = std::min(K, std::max<size_t>(1, sweep_budget_bytes() / per));
+    DeviceArray<double> d_omegas, d_z, d_x;
+    fh_status st;
+    if ((st = d_omegas.allocate(chunk)) || (st = d_z.allocate(chunk * p * p * 2)) ||
+        (st = d_x.allocate(2 * chunk * p * m)))
+      return st;
+    DenseSolveInputs in;
+    in.m = m;
+    in.p = p;
+    in.d_lmesh = s->d_lmesh.ptr;
+    in.d_r_rows = s->d_r_rows.ptr;
+    in.d_r_cols = s->d_r_cols.ptr;
+    in.d_r_vals = s->d_r_vals.ptr;
+    in.r_nnz = s->r_vals.size();
+    in.d_ree_ptr = s->d_ree_ptr.ptr;
+    in.d_ree_col = s->d_ree_col.ptr;
+    in.d_ree_val = s->d_ree_val.ptr;
+    in.d_rep = s->d_rep.ptr;
+    in.d_rpp = s->d_rpp.ptr;
+    in.d_x_out = d_x.ptr;
+    in.tight_x = true;
+    std::vector<double> omegas(chunk);
+    for (size_t c0 = 0; c0 < K; c0 += chunk) {
+      const size_t kc = std::min(chunk, K - c0);
+      for (size_t q = 0; q < kc; ++q) omegas[q] = 6.283185307179586 * freqs[c0 + q];
+      FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double), hipMemcpyHostToDevice,
+                                s->stream));
+      BatchResult res;
+      if ((st = ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws, s->stream,
+                                   nullptr)))
+        return st;
+      FH_HIP_TRY(hipMemcpyAsync(z.data() + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
+                                hipMemcpyDeviceToHost, s->stream));
+      FH_HIP_TRY(hipMemcpyAsync(X.data() + 2 * p * m * c0, d_x.ptr, 2 * kc * p * m * sizeof(double),
+                                hipMemcpyDeviceToHost, s->stream));
+      FH_HIP_TRY(hipStreamSynchronize(s->stream));
+      for (size_t q = 0; q < kc; ++q) {
+        bool bad = res.singular[q] != 0;
+        for (size_t e = 0; e < 2 * p * p && !bad; ++e)
+          if (!std::isfinite(z[2 * (c0 + q) * p * p + e])) bad = true;
+        if (bad)
+          return set_error(FH_E_SINGULAR,
+                           "the mesh system is singular or ill-conditioned at " +
+                               std::to_string(freqs[c0 + q]) + " Hz",
+                           static_cast<int64_t>(c0 + q), 0, freqs[c0 + q]);
+        if (!res.converged[q])
+          return set_error(FH_E_NOT_CONVERGED,
+                           "iterative refinement did not converge at " + std::to_string(freqs[c0 + q]) + " Hz",
+                           static_cast<int64_t>(c0 + q), res.refine_steps[q], freqs[c0 + q]);
+      }
+    }
+  }
+  // Filament currents I[k][a][f] = sum_l M[l][f] lambda_a[l].
+  std::vector<double2> I(K * p * n, make_double2(0.0, 0.0));
+  const LoopBasis &B = s->basis;
+  for (size_t k = 0; k < K; ++k)
+    for (size_t a = 0; a < p; ++a) {
+      double2 *Ia = I.data() + (k * p + a) * n;
+      const double *xr = X.data() + (2 * (k * p + a)) * m, *xi = xr + m;
+      for (size_t l = 0; l < B.loops; ++l) {
+        double lr, li;
+        if (l < m) {
+          lr = -xr[l];
+          li = -xi[l];
+        } else {
+          lr = l - m == a ? 1.0 : 0.0;
+          li = 0.0;
+        }
+        if (lr == 0.0 && li == 0.0) continue;
+        for (int64_t e = B.row_ptr[l]; e < B.row_ptr[l + 1]; ++e) {
+          double2 &c = Ia[B.branch[e]];
+          c.x += B.sign[e] * lr;
+          c.y += B.sign[e] * li;
+        }
+      }
+    }
+  DeviceArray<double2> d_I;
+  fh_status st;
+  if ((st = d_I.allocate(K * p * n))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(d_I.ptr, I.data(), K * p * n * sizeof(double2), hipMemcpyHostToDevice, s->stream));
+  // Z in the stationary form Z_ab = i_a^T (R + jw L) i_b: the same value as
+  // the Schur complement for exact currents, without its cancellation
+  // (Zpp and Zep^T x can exceed Z by orders of magnitude at low frequency),
+  // so Z(p) is smooth to rounding and finite differences of it are clean.
+  if (z_out) {
+    std::vector<double2> LI;
+    if ((st = inductance_products(s->d_L.ptr, static_cast<int>(n), d_I.ptr, static_cast<int>(K * p), LI,
+                                  s->stream)))
+      return st;
+    for (size_t k = 0; k < K; ++k) {
+      const double omega = 6.283185307179586 * freqs[k];
+      for (size_t a = 0; a < p; ++a)
+        for (size_t b = 0; b < p; ++b) {
+          const double2 *Ia = I.data() + (k * p + a) * n, *Ib = I.data() + (k * p + b) * n;
+          const double2 *Lb = LI.data() + (k * p + b) * n;
+          double rr = 0.0, ri = 0.0, lr = 0.0, li = 0.0;
+          for (size_t f = 0; f < n; ++f) {
+            const double R = s->resistances[f];
+            rr += R * (Ia[f].x * Ib[f].x - Ia[f].y * Ib[f].y);
+            ri += R * (Ia[f].x * Ib[f].y + Ia[f].y * Ib[f].x);
+            lr += Ia[f].x * Lb[f].x - Ia[f].y * Lb[f].y;
+            li += Ia[f].x * Lb[f].y + Ia[f].y * Lb[f].x;
+          }
+          // R part + jw (lr + j li)
+          z_out[2 * ((k * p + a) * p + b)] = rr - omega * li;
+          z_out[2 * ((k * p + a) * p + b) + 1] = freqs[k] == 0.0 ? 0.0 : ri + omega * lr;
+        }
+    }
+  }
+  if (P == 0) return FH_OK;
+  // dL contracted with the currents on the device, frequencies in passes of
+  // at most 128 / p^2.
+  DeviceArray<fh_filament> d_T;
+  if ((st = d_T.allocate(P * n))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(d_T.ptr, T.data(), P * n * sizeof(fh_filament), hipMemcpyHostToDevice, s->stream));
+  std::vector<double2> dLc(K * P * p * p);  // [k][q][a][b]
+  for (size_t k0 = 0; k0 < K; k0 += per_pass) {
+    const size_t kc = std::min(per_pass, K - k0);
+    std::vector<double2> c;
+    if ((st = contract_inductance_derivative(s->d_filaments.ptr, static_cast<int>(n), d_T.ptr, static_cast<int>(P),
+                                             d_I.ptr + k0 * p * n, static_cast<int>(kc), static_cast<int>(p), c,
+                                             s->stream)))
+      return st;
+    for (size_t q = 0; q < P; ++q)
+      for (size_t k = 0; k < kc; ++k)
+        for (size_t ab = 0; ab < p * p; ++ab)
+          dLc[((k0 + k) * P + q) * p * p + ab] = c[(q * kc + k) * p * p + ab];
+  }
+  // dZ = sum_f I_a I_b dR_f + jw (contraction); dR = Re dZ, dL = Im dZ / w.
+  for (size_t q = 0; q < P; ++q) {
+    std::vector<double> dr(n);
+    for (size_t f = 0; f < n; ++f) {
+      const fh_filament &F = s->filaments[f], &t = T[q * n + f];
+      dr[f] = s->resistances[f] * (t.length / F.length - t.width / F.width - t.height / F.height);
+    }
+    for (size_t k = 0; k < K; ++k) {
+      const double omega = 6.283185307179586 * freqs[k];
+      for (size_t a = 0; a < p; ++a)
+        for (size_t b = 0; b < p; ++b) {
+          const double2 *Ia = I.data() + (k * p + a) * n, *Ib = I.data() + (k * p + b) * n;
+          double re = 0.0, im = 0.0;
+          for (size_t f = 0; f < n; ++f) {
+            if (dr[f] == 0.0) continue;
+            re += dr[f] * (Ia[f].x * Ib[f].x - Ia[f].y * Ib[f].y);
+            im += dr[f] * (Ia[f].x * Ib[f].y + Ia[f].y * Ib[f].x);
+          }
+          const double2 c = dLc[(k * P + q) * p * p + a * p + b];
+          // + jw (c.x + j c.y)
+          re -= omega * c.y;
+          im += omega * c.x;
+          const size_t o = ((k * P + q) * p + a) * p + b;
+          dR_dp[o] = re;
+          dL_dp[o] = freqs[k] == 0.0 ? 0.0 : im / omega;
+        }
+    }
+  }
+  return FH_OK;
+}
+
+fh_status fh_partial_inductance_derivative(const fh_filament *filaments, size_t n, const fh_filament *tangents,
+                                           size_t n_dirs, const uint32_t *pairs, size_t count, double *value_out,
+                                           double *deriv_out) {
+  clear_error();
+  if (count == 0) return FH_OK;
+  if (!filaments || !tangents || !pairs || !value_out || !deriv_out)
+    return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (n >= (size_t(1) << 30)) return set_error(FH_E_INVALID_ARGUMENT, "too many filaments");
+  std::vector<uint4> pr(count);
+  for (size_t k = 0; k < count; ++k) {
+    const uint32_t f = pairs[3 * k], g = pairs[3 * k + 1], q = pairs[3 * k + 2];
+    if (f >= n || g >= n || q >= n_dirs)
+      return set_error(FH_E_INVALID_ARGUMENT, "pair " + std::to_string(k) + " is out of range",
+                       static_cast<int64_t>(k));
+    pr[k] = make_uint4(f, g, q, 0u);
+  }
+  fh_status st = check_device();
+  if (st) return st;
+  hipStream_t stream = nullptr;
+  FH_HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{stream};
+  DeviceArray<fh_filament> d_F, d_T;
+  DeviceArray<uint4> d_p;
+  DeviceArray<double> d_v, d_d;
+  if ((st = d_F.allocate(n)) || (st = d_T.allocate(n * n_dirs)) || (st = d_p.allocate(count)) ||
+      (st = d_v.allocate(count)) || (st = d_d.allocate(count)))
+    return st;
+  FH_HIP_TRY(hipMemcpyAsync(d_F.ptr, filaments, n * sizeof(fh_filament), hipMemcpyHostToDevice, stream));
+  FH_HIP_TRY(hipMemcpyAsync(d_T.ptr, tangents, n * n_dirs * sizeof(fh_filament), hipMemcpyHostToDevice, stream));
+  FH_HIP_TRY(hipMemcpyAsync(d_p.ptr, pr.data(), count * sizeof(uint4), hipMemcpyHostToDevice, stream));
+  if ((st = pair_derivatives(d_F.ptr, static_cast<int>(n), d_T.ptr, d_p.ptr, count, d_v.ptr, d_d.ptr, stream)))
+    return st;
+  FH_HIP_TRY(hipMemcpyAsync(value_out, d_v.ptr, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipMemcpyAsync(deriv_out, d_d.ptr, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));
+  for (size_t k = 0; k < count; ++k) {
+    if (std::isnan(value_out[k]))
+      return set_error(FH_E_KERNEL, "a sampled-filament pair has coincident sample lines", pairs[3 * k],
+                       pairs[3 * k + 1], 0.0, 1);
+    if (std::isnan(deriv_out[k]))
+      return set_error(FH_E_UNSUPPORTED,
+                       "pair " + std::to_string(k) +
+                           ": the direction turns a perpendicular or parallel filament pair relative to each other",
+                       pairs[3 * k], pairs[3 * k + 1]);
+  }
+  return FH_OK;
 }
 
 }  // extern "C"

@@ -11,6 +11,9 @@
   R(DC) = 2.844828 ohm (docs/validation.md:107-112).
 * ``batch_problems`` -- the batched-sweep workload: seeded square spirals and
   busbar pairs of uneven sizes (300 - 4 000 filaments), surface-graded.
+* ``AdjointSpiral`` -- the gradient workload: a planar square spiral whose
+  turn widths and spacings are the parameters, its discretization frozen at
+  the nominal geometry, with the exact node and width velocities.
 """
 from __future__ import annotations
 
@@ -19,7 +22,8 @@ from typing import List, Tuple
 
 import numpy as np
 
-from .model import (Discretization, Geometry, Node, Port, SegmentDef, Subdivision)
+from .model import (AxisGrading, Discretization, Geometry, Node, Port, SegmentDef,
+                    SkinDepthGrading, Subdivision)
 
 COPPER = 5.8e7
 
@@ -292,3 +296,103 @@ def batch_problems(count: int = 48, seed: int = 48):
             per_bar = max(1, int(round((target / (nw * nh) - 1) / 2)))
             out.append(graded_busbar_pair(width, spacing, thickness, nw, nh, per_bar))
     return out
+
+
+class AdjointSpiral:
+    """A planar square spiral parameterized by its turn widths w_k and
+    spacings s_k (p = [w_0 .. w_{T-1}, s_0 .. s_{T-1}]), outermost turn first.
+
+    The outer edge is a square of side ``outer`` centred on the origin, in
+    the plane z = thickness / 2.  Turn k's centreline sits a_k = sum_{i<k}
+    (w_i + s_i) + w_k / 2 in from that square; its top, right and bottom sides
+    run at that offset and its left side climbs to the next turn's top side
+    (the last one to a_{T-1} + w_{T-1} + s_{T-1}, so every spacing counts).
+    One port across the outer and inner ends.
+
+    The discretization is frozen at the nominal parameters: side j is cut
+    into ``splits[j]`` segments (about equal lengths, giving about
+    ``filaments`` filaments) and turn k's segments into the skin-depth counts
+    of SkinDepthGrading(frequency, target, ratio) for w_k x thickness, as
+    per-segment graded grids.  Node positions are affine in p, so
+    ``directions()`` returns exact velocities."""
+
+    def __init__(self, turns: int = 6, outer: float = 2e-3, width: float = 100e-6,
+                 spacing: float = 50e-6, thickness: float = 35e-6, frequency: float = 1e9,
+                 target: float = 0.5, ratio: float = 2.0, filaments: int = 6000):
+        self.turns = turns
+        self.outer = outer
+        self.thickness = thickness
+        self.nominal = np.array([width] * turns + [spacing] * turns, dtype=np.float64)
+        grading = SkinDepthGrading(frequency, target, ratio)
+        self.counts = [(grading.count_for(width, COPPER), grading.count_for(thickness, COPPER))] * turns
+        self.ratio = ratio
+        corners = self.corners(self.nominal)
+        lengths = [math.hypot(b[0] - a[0], b[1] - a[1]) for a, b in zip(corners[:-1], corners[1:])]
+        per = [self.counts[j // 4][0] * self.counts[j // 4][1] for j in range(len(lengths))]
+        segments = filaments / (sum(per) / len(per))
+        step = sum(lengths) / segments
+        self.splits = [max(1, int(round(L / step))) for L in lengths]
+
+    @property
+    def n_params(self) -> int:
+        return 2 * self.turns
+
+    def corners(self, params) -> List[Tuple[float, float]]:
+        w = [float(x) for x in params[:self.turns]]
+        sp = [float(x) for x in params[self.turns:]]
+        h = 0.5 * self.outer
+        a, acc = [], 0.0
+        for k in range(self.turns):
+            a.append(acc + 0.5 * w[k])
+            acc += w[k] + sp[k]
+        a.append(a[-1] + w[-1] + sp[-1])
+        pts = [(-h + a[0], h - a[0])]
+        for k in range(self.turns):
+            pts += [(h - a[k], h - a[k]), (h - a[k], -h + a[k]), (-h + a[k], -h + a[k]),
+                    (-h + a[k], h - a[k + 1])]
+        return pts
+
+    def node_positions(self, params) -> np.ndarray:
+        pts = self.corners(params)
+        out = [(pts[0][0], pts[0][1])]
+        for j, (a, b) in enumerate(zip(pts[:-1], pts[1:])):
+            n = self.splits[j]
+            for q in range(1, n + 1):
+                t = q / n
+                out.append((a[0] + t * (b[0] - a[0]), a[1] + t * (b[1] - a[1])))
+        xyz = np.zeros((len(out), 3))
+        xyz[:, :2] = out
+        xyz[:, 2] = 0.5 * self.thickness
+        return xyz
+
+    def segment_turns(self) -> List[int]:
+        return [j // 4 for j, n in enumerate(self.splits) for _ in range(n)]
+
+    def problem(self, params=None):
+        """(geometry, ports, discretization) at ``params`` (default nominal)."""
+        p = self.nominal if params is None else np.asarray(params, dtype=np.float64)
+        xyz = self.node_positions(p)
+        nodes = [Node(*map(float, r)) for r in xyz]
+        turn = self.segment_turns()
+        defs = [SegmentDef(i, i + 1, float(p[turn[i]]), self.thickness, COPPER)
+                for i in range(len(turn))]
+        g = Geometry.from_parts(nodes, defs)
+        grids = [AxisGrading(self.counts[k][0], self.counts[k][1], self.ratio, self.ratio)
+                 for k in turn]
+        return g, [Port(0, len(nodes) - 1, "spiral")], Discretization.per_segment_graded(grids)
+
+    def directions(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(d_nodes (P, n_nodes, 3), d_widths (P, n_segments)): node positions
+        are affine in p, so a unit step gives the exact velocity."""
+        base = self.node_positions(self.nominal)
+        P = self.n_params
+        d_nodes = np.zeros((P,) + base.shape)
+        for q in range(P):
+            e = self.nominal.copy()
+            e[q] += 1.0
+            d_nodes[q] = self.node_positions(e) - base
+        turn = np.array(self.segment_turns())
+        d_widths = np.zeros((P, len(turn)))
+        for k in range(self.turns):
+            d_widths[k, turn == k] = 1.0
+        return d_nodes, d_widths

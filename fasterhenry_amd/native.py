@@ -30,6 +30,7 @@ SYMBOLS = [
     "fh_error_detail", "fh_version", "fh_iterative_default_params", "fh_system_create_iterative",
     "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
     "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
+    "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative",
 ]
 
 _ERRORS = {
@@ -60,6 +61,12 @@ class SweepInfo(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ParamMap(C.Structure):
+    """fh_param_map: geometry directions of the gradient parameters."""
+    _fields_ = [("n_params", C.c_size_t), ("d_nodes", C.c_void_p), ("d_widths", C.c_void_p),
+                ("d_heights", C.c_void_p)]
 
 
 class PfftParams(C.Structure):
@@ -167,6 +174,12 @@ def lib() -> C.CDLL:
         L.fh_pfft_data.argtypes = [vp] * 7
         L.fh_pfft_destroy.restype = None
         L.fh_pfft_destroy.argtypes = [vp]
+        L.fh_discretize_tangent.restype = st
+        L.fh_discretize_tangent.argtypes = [C.POINTER(Problem), vp, vp, vp, vp, sz]
+        L.fh_grad.restype = st
+        L.fh_grad.argtypes = [vp, C.POINTER(ParamMap), vp, sz, vp, vp, vp]
+        L.fh_partial_inductance_derivative.restype = st
+        L.fh_partial_inductance_derivative.argtypes = [vp, sz, vp, sz, vp, sz, vp, vp]
         L.fh_solve_batch.restype = st
         L.fh_solve_batch.argtypes = [vp, sz, C.c_int, C.c_int, vp, sz, vp, vp, C.c_int32,
                                      C.POINTER(sz)]

@@ -259,6 +259,54 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
                          fh_precision precision, const double *freqs_hz, size_t K, double *z_out,
                          fh_sweep_info *infos, int32_t concurrency, size_t *failed);
 
+/* Geometry parameters for gradients: how the problem moves along each of
+ * n_params directions.  d_nodes: n_params x n_nodes x 3 (d node xyz / dp);
+ * d_widths, d_heights: n_params x n_segments (d segment width / dp, d height
+ * / dp), either may be NULL (no change).  Filament counts and grading
+ * ratios are held at the problem's (frozen discretization). */
+typedef struct fh_param_map {
+  size_t n_params;
+  const double *d_nodes;
+  const double *d_widths;
+  const double *d_heights;
+} fh_param_map;
+
+/* Gradients of the port impedance.  No counterpart in the reference (it has
+ * no derivatives); the north star's adjoint: for each frequency k, parameter
+ * q and port pair (a, b), dR_dp = Re dZ_ab/dp_q and dL_dp = Im dZ_ab/dp_q / w
+ * (0 at f = 0), both K x n_params x p x p row-major; z_out (may be NULL)
+ * receives Z (K x p x p {re, im}) in the stationary form i_a^T (R + jw L) i_b
+ * of the refined currents: fh_sweep's Z to within its tolerance, without the
+ * Schur complement's cancellation.  Z is stationary in the loop currents, so
+ * dZ_ab = i_a^T (dR + jw dL) i_b with i_a the filament currents of port a's
+ * solution: one refined solve per frequency, then the derivative kernels of
+ * the partial inductances (methods, quadrature orders and subdivisions held
+ * at the nominal geometry) contracted with the currents on the device.
+ * Dense systems only. */
+fh_status fh_grad(fh_system *system, const fh_param_map *map, const double *freqs_hz, size_t K,
+                  double *z_out, double *dR_dp, double *dL_dp);
+
+/* Host level: d(filaments)/dp along one direction (d_nodes n_nodes x 3,
+ * d_widths / d_heights n_segments or NULL): out (n filaments) receives
+ * d(start), d(end), d(length), d(dir), d(wdir), d(hdir), d(width),
+ * d(height) of each filament, sigma 0.  For checking fh_grad's geometry
+ * derivatives against finite differences of fh_discretize. */
+fh_status fh_discretize_tangent(const fh_problem *problem, const double *d_nodes,
+                                const double *d_widths, const double *d_heights, fh_filament *out,
+                                size_t capacity);
+
+/* Kernel level, gradients: nominal partial inductances and their
+ * derivatives along filament tangents, pair by pair.  No counterpart in the
+ * reference.  tangents: n_dirs x n filaments (as fh_discretize_tangent
+ * writes them); pairs: count x {f, g, direction}.  value_out[k] is L_fg
+ * (the same bits as fh_partial_inductance's entry), deriv_out[k] its
+ * derivative along the direction with the pair's method, quadrature orders
+ * and aligned-quadrature subdivision held at the nominal geometry. */
+fh_status fh_partial_inductance_derivative(const fh_filament *filaments, size_t n,
+                                           const fh_filament *tangents, size_t n_dirs,
+                                           const uint32_t *pairs, size_t count, double *value_out,
+                                           double *deriv_out);
+
 /* Kernel level.  partial_inductance_matrix (inductance/batch.rs:246-253):
  * L_out (n x n, row-major, host) for host filaments.  methods (may be NULL)
  * receives the Method of inductance/mod.rs:130-152 per entry (0 orthogonal,
