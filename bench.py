@@ -214,6 +214,99 @@ def batch(steps: int, warmup: int) -> dict:
     }
 
 
+def adjoint(steps: int, warmup: int, fd_check: bool = True, cpu: bool = False) -> dict:
+    """The gradient workload: a 6-turn square spiral (outer 2 mm, w 100 um,
+    s 50 um, t 35 um), skin-depth graded at 1 GHz (0.5 delta, ratio 2) and
+    frozen, 6 156 filaments; 12 parameters (per-turn width and spacing);
+    dR/dp and dL/dp at 1, 10, 100, 500 MHz and 1 GHz through fh_grad (refined
+    solve, filament currents, derivative fill contracted on the device).
+    Checked against central differences of Z at relative step 1e-6 (Z from
+    the same refined solve in its stationary form, 24 perturbed assemblies)."""
+    from fasterhenry_amd import MeshSystem, fixtures
+
+    freqs = [1e6, 1e7, 1e8, 5e8, 1e9]
+    s = fixtures.AdjointSpiral()
+    g, ports, d = s.problem()
+    t0 = time.perf_counter()
+    system = MeshSystem.assemble(g, ports, d, precision="fp32_refined")
+    assembly_s = time.perf_counter() - t0
+    counts = system.counts()
+    d_nodes, d_widths = s.directions()
+    print(f"# adjoint: {counts.filaments} filaments, {s.n_params} params", flush=True)
+    for _ in range(warmup):
+        system.grad(freqs, d_nodes, d_widths)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        z, dr, dl = system.grad(freqs, d_nodes, d_widths)
+        print(f"# adjoint step {k + 1}/{steps} done", flush=True)
+    elapsed = (time.perf_counter() - t0) / steps
+    system.close()
+    out = {
+        "metric": "adjoint gradients/s (12 params x 5 freqs, dR/dp and dL/dp)",
+        "value": 1.0 / elapsed,
+        "unit": "gradients/s",
+        "ms_per_step": elapsed * 1e3,
+        "dtype": "fp64 derivative fill + fp32 split-bf16 factorization + fp64 refinement",
+        "config": {"workload": f"6-turn square spiral 2 mm, w 100 um s 50 um t 35 um, skin-depth "
+                               f"graded (1 GHz, 0.5 delta, ratio 2) frozen, {counts.filaments} "
+                               f"filaments, 12 params, 5 freqs 1 MHz-1 GHz",
+                   "filaments": counts.filaments, "internal_loops": counts.internal_meshes,
+                   "params": s.n_params, "frequencies": len(freqs), "assembly_s": assembly_s},
+    }
+    if fd_check:
+        w = 2 * np.pi * np.asarray(freqs)
+        fr = np.zeros((len(freqs), s.n_params))
+        fl = np.zeros_like(fr)
+        t1 = time.perf_counter()
+        for q in range(s.n_params):
+            h = 1e-6 * s.nominal[q]
+            zz = []
+            for sgn in (1.0, -1.0):
+                p = s.nominal.copy()
+                p[q] += sgn * h
+                sq = MeshSystem.assemble(*s.problem(p), precision="fp32_refined")
+                zz.append(sq.grad(freqs, np.zeros((0,) + d_nodes.shape[1:]))[0][:, 0, 0])
+                sq.close()
+            dz = (zz[0] - zz[1]) / (2 * h)
+            fr[:, q], fl[:, q] = dz.real, dz.imag / w
+        fd_s = time.perf_counter() - t1
+        er = np.abs(dr[:, :, 0, 0] - fr).max(axis=1) / np.abs(fr).max(axis=1)
+        el = np.abs(dl[:, :, 0, 0] - fl).max(axis=1) / np.abs(fl).max(axis=1)
+        out["fd_check"] = {"step": "central, relative 1e-6", "dR_rel_err_max": float(er.max()),
+                           "dL_rel_err_max": float(el.max()),
+                           "dR_rel_err_per_freq": [float(x) for x in er],
+                           "dL_rel_err_per_freq": [float(x) for x in el],
+                           "error_norm": "max over params of |adjoint - fd| / max over params |fd|, "
+                                         "per frequency",
+                           "fd_gradient_s_on_gpu": fd_s}
+    if cpu:
+        import oracle
+
+        threads = os.cpu_count() or 1
+        print(f"# adjoint cpu baseline: one oracle assembly + one frequency, {threads} threads",
+              flush=True)
+        t1 = time.perf_counter()
+        osys = oracle.System(model_pack(s.problem()), threads)
+        ta = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        osys.sweep([freqs[2]])
+        tf = time.perf_counter() - t1
+        per_grad = (2 * s.n_params + 1) * (ta + len(freqs) * tf)
+        out["cpu_baseline"] = {
+            "value": 1.0 / per_grad, "unit": "gradients/s", "cores": threads, "kind": "port",
+            "sample": f"one oracle assembly ({ta:.2f} s) and one frequency solve ({tf:.2f} s) of "
+                      f"the same spiral; a CPU gradient by central differences takes "
+                      f"{2 * s.n_params + 1} assemblies x (assembly + {len(freqs)} solves) = "
+                      f"{per_grad:.1f} s on {threads} threads of {cpu_model()}"}
+    return out
+
+
+def model_pack(problem):
+    from fasterhenry_amd import model
+
+    return model.pack(*problem)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,13 +318,17 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 sweep the refined result is compared with")
-    ap.add_argument("--workload", default="dense", choices=["dense", "matrix-free", "batch"],
-                    help="dense: the headline line (with the matrix-free and batched sweeps "
-                         "attached); matrix-free / batch: only that sweep, as its own line")
+    ap.add_argument("--workload", default="dense", choices=["dense", "matrix-free", "batch", "adjoint"],
+                    help="dense: the headline line (with the matrix-free, batched and adjoint sweeps "
+                         "attached); matrix-free / batch / adjoint: only that workload, as its own line")
     ap.add_argument("--no-matrix-free", action="store_true",
                     help="leave the matrix-free sweep out of the headline line")
     ap.add_argument("--no-batch", action="store_true",
                     help="leave the batched sweep out of the headline line")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="leave the adjoint gradients out of the headline line")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="adjoint workload: skip the central-difference check")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
@@ -246,6 +343,10 @@ def main() -> int:
         return 0
     if args.workload == "batch":
         out = batch(args.steps, args.warmup)
+        print(json.dumps(out))
+        return 0
+    if args.workload == "adjoint":
+        out = adjoint(args.steps, args.warmup, fd_check=not args.no_fd, cpu=not args.no_cpu)
         print(json.dumps(out))
         return 0
 
@@ -349,6 +450,8 @@ def main() -> int:
         out["matrix_free"] = matrix_free(1, 1, dense_check=(packed, ports))
     if not args.no_batch:
         out["batch"] = batch(1, 1)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint(1, 1)
     print(json.dumps(out))
     system.close()
     return 0
