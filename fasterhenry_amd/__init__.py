@@ -10,7 +10,9 @@ solve loads the native library and raises if it or the device is missing.
 from .errors import *  # noqa: F401,F403
 from .model import *  # noqa: F401,F403
 from .model import DENSE_PATH_MAX_FILAMENTS, MU0  # noqa: F401
-from .api import (Counts, IterativeSystem, MeshSystem, PfftOperator, SweepResult,  # noqa: F401
+from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # noqa: F401
+                  IterativeSystem, MeshSystem, PfftOperator, PowerBalance, SegmentCurrent, SweepResult,
+                  unit_excitation,
                   Timing, device_math, discretize_geometry, discretize_packed, discretize_tangent,
                   gmres_dense, iterative_params, partial_inductance_derivative,
                   partial_inductance_matrix, pfft_params, solve, solve_batch, version)

@@ -103,6 +103,120 @@ class SweepResult:
         return json.dumps(doc)
 
 
+@dataclass
+class FilamentCurrent:
+    """Current in one filament (currents.rs:54-71)."""
+
+    filament: int
+    segment: int
+    conductor: int
+    location: List[float]
+    current: complex
+    current_density: float
+    power_w: float
+
+
+@dataclass
+class SegmentCurrent:
+    """Total current through, and power in, one segment (currents.rs:74-84)."""
+
+    segment: int
+    conductor: int
+    current: complex
+    power_w: float
+
+
+@dataclass
+class ConductorPeak:
+    """The hottest filament of one conductor (currents.rs:87-97)."""
+
+    conductor: int
+    peak_current_density: float
+    filament: int
+    location: List[float]
+
+
+@dataclass
+class PowerBalance:
+    """Dissipated versus delivered power (currents.rs:100-108)."""
+
+    dissipated_w: float
+    input_w: float
+    relative_residual: float
+
+
+@dataclass
+class CurrentSolution:
+    """All currents of one frequency and excitation (currents.rs:111-124):
+    peak phasors in amperes, |J| in A/m^2, time-average power in watts."""
+
+    frequency_hz: float
+    excitation: List[complex]
+    filaments: List[FilamentCurrent]
+    segments: List[SegmentCurrent]
+    conductors: List[ConductorPeak]
+    power_balance: PowerBalance
+
+    def to_dict(self) -> dict:
+        def c(z):
+            return {"re": float(z.real), "im": float(z.imag)}
+        return {
+            "frequency_hz": self.frequency_hz,
+            "excitation": [c(z) for z in self.excitation],
+            "filaments": [dict(filament=f.filament, segment=f.segment, conductor=f.conductor,
+                               location=f.location, current=c(f.current),
+                               current_density=f.current_density, power_w=f.power_w)
+                          for f in self.filaments],
+            "segments": [dict(segment=g.segment, conductor=g.conductor, current=c(g.current),
+                              power_w=g.power_w) for g in self.segments],
+            "conductors": [dict(conductor=k.conductor, peak_current_density=k.peak_current_density,
+                                filament=k.filament, location=k.location) for k in self.conductors],
+            "power_balance": dict(dissipated_w=self.power_balance.dissipated_w,
+                                  input_w=self.power_balance.input_w,
+                                  relative_residual=self.power_balance.relative_residual),
+        }
+
+
+def unit_excitation(ports: int) -> List[complex]:
+    """1 A into port 1, every other port open (currents.rs:126-132)."""
+    return [complex(1.0 if i == 0 else 0.0, 0.0) for i in range(ports)]
+
+
+def _currents(handle, counts, packed: PackedProblem, filaments: np.ndarray, frequency: float,
+              excitation) -> CurrentSolution:
+    ex = np.ascontiguousarray(np.asarray(excitation, dtype=np.complex128).reshape(-1))
+    n, nseg, p = counts.filaments, counts.segments, counts.ports
+    cur = np.zeros(n, dtype=np.complex128)
+    dens = np.zeros(n)
+    power = np.zeros(n)
+    scur = np.zeros(nseg, dtype=np.complex128)
+    spow = np.zeros(nseg)
+    cond = np.zeros(nseg, dtype=np.int32)
+    peaks = np.zeros(max(nseg, 1), dtype=np.int64)
+    volt = np.zeros(max(p, 1), dtype=np.complex128)
+    ncond = C.c_size_t()
+    bal = native.PowerBalance()
+    native.check(native.lib().fh_currents(
+        handle, float(frequency), native.ptr(ex) if len(ex) else None, len(ex), native.ptr(cur),
+        native.ptr(dens), native.ptr(power), native.ptr(scur), native.ptr(spow), native.ptr(cond),
+        C.byref(ncond), native.ptr(peaks), native.ptr(volt), C.byref(bal)))
+    # Filament::center (filament.rs:91-93)
+    centre = (np.asarray(filaments["start"]) + np.asarray(filaments["end"])) * 0.5
+    # filaments come segment by segment
+    per = packed.grids["nw"].astype(np.int64) * packed.grids["nh"].astype(np.int64)
+    seg_of = np.repeat(np.arange(nseg), per)
+    fils = []
+    for k in range(n):
+        q = int(seg_of[k])
+        fils.append(FilamentCurrent(k, q, int(cond[q]), [float(x) for x in centre[k]], complex(cur[k]),
+                                    float(dens[k]), float(power[k])))
+    segs = [SegmentCurrent(q, int(cond[q]), complex(scur[q]), float(spow[q])) for q in range(nseg)]
+    conds = [ConductorPeak(c, float(dens[peaks[c]]), int(peaks[c]),
+                           [float(x) for x in centre[peaks[c]]]) for c in range(ncond.value)]
+    return CurrentSolution(float(frequency), [complex(z) for z in ex], fils, segs, conds,
+                           PowerBalance(bal.dissipated_w, bal.input_w, bal.relative_residual))
+
+
 def _counts(handle) -> Counts:
     c = (C.c_size_t * 6)()
     native.check(native.lib().fh_system_counts(handle, c))
@@ -190,6 +304,13 @@ class MeshSystem:
         out = np.zeros((n, n))
         native.check(native.lib().fh_system_inductance(self._h, native.ptr(out)))
         return out
+
+    def currents(self, frequency: float, excitation) -> CurrentSolution:
+        """Filament currents, |J| and power for port-current phasors
+        ``excitation`` (MeshSystem::currents, solve.rs:835-868)."""
+        if getattr(self, "_fil", None) is None:
+            self._fil = self.filaments()
+        return _currents(self._h, self._counts, self._packed, self._fil, frequency, excitation)
 
     def grad(self, frequencies, d_nodes, d_widths=None, d_heights=None) -> tuple:
         """Z and its derivatives along n_params geometry directions (fh_grad;
@@ -319,6 +440,13 @@ class IterativeSystem:
     def impedance(self, frequency: float) -> np.ndarray:
         """Z at one frequency (iterative.rs:270-272)."""
         return self.sweep_raw([frequency])[0][0]
+
+    def currents(self, frequency: float, excitation) -> CurrentSolution:
+        """Filament currents, |J| and power for port-current phasors
+        ``excitation`` (IterativeSystem::currents, iterative.rs:329-380)."""
+        if getattr(self, "_fil", None) is None:
+            self._fil = self.filaments()
+        return _currents(self._h, self._counts, self._packed, self._fil, frequency, excitation)
 
 
 class PfftOperator:

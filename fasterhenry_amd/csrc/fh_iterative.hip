@@ -747,4 +747,64 @@ fh_status IterativeSystem::impedance(double frequency, double *z, std::vector<si
   return FH_OK;
 }
 
+fh_status IterativeSystem::currents(double frequency, const double *excitation, std::vector<double> &internal,
+                                    std::vector<double> &voltage, GmresOutcome &outcome) {
+  const size_t m = m_, p = p_;
+  const bool dc = frequency == 0.0;
+  const double omega = 6.283185307179586 * frequency;
+  auto ep = [&](size_t i, size_t a) { return cd(r_ep_[i * p + a], dc ? 0.0 : omega * l_ep_[i * p + a]); };
+  std::vector<cd> ex(p), x(m);
+  bool complex_ex = false;
+  for (size_t a = 0; a < p; ++a) {
+    ex[a] = cd(excitation[2 * a], excitation[2 * a + 1]);
+    complex_ex = complex_ex || ex[a].imag() != 0.0;
+  }
+  outcome = GmresOutcome{};
+  if (m > 0) {
+    std::vector<double2> diag(m), rhs(m);
+    for (size_t i = 0; i < m; ++i) {
+      diag[i] = make_double2(r_diag_[i], dc ? 0.0 : omega * l_diag_[i]);
+      cd acc(0.0, 0.0);
+      for (size_t a = 0; a < p; ++a) acc = hadd(acc, hmul(ep(i, a), ex[a]));
+      rhs[i] = make_double2(acc.real(), acc.imag());
+    }
+    FH_HIP_TRY(hipMemcpyAsync(d_diag_.ptr, diag.data(), m * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    FH_HIP_TRY(hipMemcpyAsync(d_rhs_.ptr, rhs.data(), m * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    zero2_kernel<<<grid_for(m), kThreads, 0, stream_>>>(d_x_.ptr, m);
+    FH_HIP_TRY(hipGetLastError());
+    auto apply = [&](const double2 *in, double2 *out) { return apply_ee(omega, dc, in, out); };
+    // the real form at DC for a real excitation (every imaginary part zero)
+    fh_status st = solver_.solve(apply, d_diag_.ptr, d_rhs_.ptr, d_x_.ptr, m, gmres_, !dc || complex_ex, stream_,
+                                 outcome);
+    if (st) return st;
+    if (!outcome.converged)
+      return set_error(FH_E_NOT_CONVERGED,
+                       "GMRES did not converge at " + std::to_string(frequency) + " Hz: relative residual " +
+                           std::to_string(outcome.relative_residual) + " after " +
+                           std::to_string(outcome.iterations) + " iterations",
+                       0, 0, outcome.relative_residual,
+                       static_cast<int32_t>(std::min<size_t>(outcome.iterations, INT32_MAX)));
+    FH_HIP_TRY(hipMemcpyAsync(rhs.data(), d_x_.ptr, m * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    FH_HIP_TRY(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < m; ++i) x[i] = cd(-rhs[i].x, -rhs[i].y);
+  }
+  internal.assign(2 * m, 0.0);
+  for (size_t i = 0; i < m; ++i) {
+    internal[2 * i] = x[i].real();
+    internal[2 * i + 1] = x[i].imag();
+  }
+  voltage.assign(2 * p, 0.0);
+  for (size_t a = 0; a < p; ++a) {
+    cd acc(0.0, 0.0);
+    for (size_t b = 0; b < p; ++b)
+      acc = hadd(acc, hmul(cd(r_pp_[a * p + b], dc ? 0.0 : omega * l_pp_[a * p + b]), ex[b]));
+    cd tr(0.0, 0.0);
+    for (size_t i = 0; i < m; ++i) tr = hadd(tr, hmul(ep(i, a), x[i]));
+    acc = hadd(acc, tr);
+    voltage[2 * a] = acc.real();
+    voltage[2 * a + 1] = acc.imag();
+  }
+  return FH_OK;
+}
+
 }  // namespace fh

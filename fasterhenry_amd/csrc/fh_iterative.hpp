@@ -54,6 +54,11 @@ class IterativeSystem {
   // z: p x p complex {re, im} row-major (host).  iterations: p entries.
   fh_status impedance(double frequency, double *z, std::vector<size_t> &iterations,
                       double &residual_max);
+  // solve_currents (solve.rs:1110-1149) with GMRES (iterative.rs:329-380):
+  // excitation p complex {re, im}; internal (m) = -Zee^-1 Zep I_p and the
+  // port voltages (p) Zpp I_p + Zep^T internal, complex {re, im}.
+  fh_status currents(double frequency, const double *excitation, std::vector<double> &internal,
+                     std::vector<double> &voltage, GmresOutcome &outcome);
   const PfftOperator &op() const { return op_; }
   PfftOperator &op() { return op_; }
   double operator_seconds() const { return operator_s_; }

@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -223,6 +224,68 @@ size_t sweep_budget_bytes() {
   // Leave headroom; never ask for more than 96 GiB of workspace.
   size_t b = free_b / 2;
   return std::min(b, size_t(96) << 30);
+}
+
+
+// x = Zee^-1 Zep (2 K p vectors of m: real then imaginary part of column a
+// at frequency k, vector 2 (k p + a)) by the refined solve, refined until
+// the correction is at the rounding level (tight_x); first failure in list
+// order as in fh_sweep.  The system must be dense; nothing to solve for m = 0.
+fh_status refined_x(fh_system *s, const double *freqs, size_t K, std::vector<double> &X) {
+  const size_t m = s->basis.internal, p = s->basis.loops - m;
+  X.assign(2 * K * p * m, 0.0);
+  if (m == 0 || K == 0) return FH_OK;
+  const size_t per = ldlt_bytes_per_frequency(m, p, true) + 2 * p * m * sizeof(double);
+  const size_t chunk = std::min(K, std::max<size_t>(1, sweep_budget_bytes() / per));
+  DeviceArray<double> d_omegas, d_z, d_x;
+  fh_status st;
+  if ((st = d_omegas.allocate(chunk)) || (st = d_z.allocate(chunk * p * p * 2)) ||
+      (st = d_x.allocate(2 * chunk * p * m)))
+    return st;
+  DenseSolveInputs in;
+  in.m = m;
+  in.p = p;
+  in.d_lmesh = s->d_lmesh.ptr;
+  in.d_r_rows = s->d_r_rows.ptr;
+  in.d_r_cols = s->d_r_cols.ptr;
+  in.d_r_vals = s->d_r_vals.ptr;
+  in.r_nnz = s->r_vals.size();
+  in.d_ree_ptr = s->d_ree_ptr.ptr;
+  in.d_ree_col = s->d_ree_col.ptr;
+  in.d_ree_val = s->d_ree_val.ptr;
+  in.d_rep = s->d_rep.ptr;
+  in.d_rpp = s->d_rpp.ptr;
+  in.d_x_out = d_x.ptr;
+  in.tight_x = true;
+  std::vector<double> omegas(chunk), z(2 * chunk * p * p);
+  for (size_t c0 = 0; c0 < K; c0 += chunk) {
+    const size_t kc = std::min(chunk, K - c0);
+    for (size_t q = 0; q < kc; ++q) omegas[q] = 6.283185307179586 * freqs[c0 + q];
+    FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    BatchResult res;
+    if ((st = ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws, s->stream, nullptr)))
+      return st;
+    FH_HIP_TRY(hipMemcpyAsync(z.data(), d_z.ptr, kc * p * p * 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    FH_HIP_TRY(hipMemcpyAsync(X.data() + 2 * p * m * c0, d_x.ptr, 2 * kc * p * m * sizeof(double),
+                              hipMemcpyDeviceToHost, s->stream));
+    FH_HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t q = 0; q < kc; ++q) {
+      bool bad = res.singular[q] != 0;
+      for (size_t e = 0; e < 2 * p * p && !bad; ++e)
+        if (!std::isfinite(z[2 * q * p * p + e])) bad = true;
+      for (size_t e = 0; e < 2 * p * m && !bad; ++e)
+        if (!std::isfinite(X[2 * p * m * (c0 + q) + e])) bad = true;
+      if (bad)
+        return set_error(FH_E_SINGULAR,
+                         "the mesh system is singular or ill-conditioned at " + std::to_string(freqs[c0 + q]) + " Hz",
+                         static_cast<int64_t>(c0 + q), 0, freqs[c0 + q]);
+      if (!res.converged[q])
+        return set_error(FH_E_NOT_CONVERGED,
+                         "iterative refinement did not converge at " + std::to_string(freqs[c0 + q]) + " Hz",
+                         static_cast<int64_t>(c0 + q), res.refine_steps[q], freqs[c0 + q]);
+    }
+  }
+  return FH_OK;
 }
 
 }  // namespace
@@ -916,65 +979,11 @@ fh_status fh_grad(fh_system *s, const fh_param_map *map, const double *freqs, si
     std::copy(t.begin(), t.end(), T.begin() + q * n);
   }
   // Loop currents: x = Zee^-1 Zep of every frequency, refined to working
-  // accuracy (they enter the gradient to first order), and Z itself.
-  std::vector<double> z(2 * K * p * p);
-  std::vector<double> X(2 * K * p * m);
-  if (m == 0) {
-    fh_status st = fh_sweep(s, freqs, K, z.data(), nullptr);
+  // accuracy (they enter the gradient to first order).
+  std::vector<double> X;
+  {
+    fh_status st = refined_x(s, freqs, K, X);
     if (st) return st;
-  } else {
-    const size_t per = ldlt_bytes_per_frequency(m, p, true) + 2 * p * m * sizeof(double);
-    const size_t chunk = std::min(K, std::max<size_t>(1, sweep_budget_bytes() / per));
-    DeviceArray<double> d_omegas, d_z, d_x;
-    fh_status st;
-    if ((st = d_omegas.allocate(chunk)) || (st = d_z.allocate(chunk * p * p * 2)) ||
-        (st = d_x.allocate(2 * chunk * p * m)))
-      return st;
-    DenseSolveInputs in;
-    in.m = m;
-    in.p = p;
-    in.d_lmesh = s->d_lmesh.ptr;
-    in.d_r_rows = s->d_r_rows.ptr;
-    in.d_r_cols = s->d_r_cols.ptr;
-    in.d_r_vals = s->d_r_vals.ptr;
-    in.r_nnz = s->r_vals.size();
-    in.d_ree_ptr = s->d_ree_ptr.ptr;
-    in.d_ree_col = s->d_ree_col.ptr;
-    in.d_ree_val = s->d_ree_val.ptr;
-    in.d_rep = s->d_rep.ptr;
-    in.d_rpp = s->d_rpp.ptr;
-    in.d_x_out = d_x.ptr;
-    in.tight_x = true;
-    std::vector<double> omegas(chunk);
-    for (size_t c0 = 0; c0 < K; c0 += chunk) {
-      const size_t kc = std::min(chunk, K - c0);
-      for (size_t q = 0; q < kc; ++q) omegas[q] = 6.283185307179586 * freqs[c0 + q];
-      FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double), hipMemcpyHostToDevice,
-                                s->stream));
-      BatchResult res;
-      if ((st = ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws, s->stream,
-                                   nullptr)))
-        return st;
-      FH_HIP_TRY(hipMemcpyAsync(z.data() + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
-                                hipMemcpyDeviceToHost, s->stream));
-      FH_HIP_TRY(hipMemcpyAsync(X.data() + 2 * p * m * c0, d_x.ptr, 2 * kc * p * m * sizeof(double),
-                                hipMemcpyDeviceToHost, s->stream));
-      FH_HIP_TRY(hipStreamSynchronize(s->stream));
-      for (size_t q = 0; q < kc; ++q) {
-        bool bad = res.singular[q] != 0;
-        for (size_t e = 0; e < 2 * p * p && !bad; ++e)
-          if (!std::isfinite(z[2 * (c0 + q) * p * p + e])) bad = true;
-        if (bad)
-          return set_error(FH_E_SINGULAR,
-                           "the mesh system is singular or ill-conditioned at " +
-                               std::to_string(freqs[c0 + q]) + " Hz",
-                           static_cast<int64_t>(c0 + q), 0, freqs[c0 + q]);
-        if (!res.converged[q])
-          return set_error(FH_E_NOT_CONVERGED,
-                           "iterative refinement did not converge at " + std::to_string(freqs[c0 + q]) + " Hz",
-                           static_cast<int64_t>(c0 + q), res.refine_steps[q], freqs[c0 + q]);
-      }
-    }
   }
   // Filament currents I[k][a][f] = sum_l M[l][f] lambda_a[l].
   std::vector<double2> I(K * p * n, make_double2(0.0, 0.0));
@@ -1130,6 +1139,167 @@ fh_status fh_partial_inductance_derivative(const fh_filament *filaments, size_t 
                        "pair " + std::to_string(k) +
                            ": the direction turns a perpendicular or parallel filament pair relative to each other",
                        pairs[3 * k], pairs[3 * k + 1]);
+  }
+  return FH_OK;
+}
+
+fh_status fh_currents(fh_system *s, double frequency, const double *excitation, size_t n_excitation,
+                      double *filament_current, double *current_density, double *power_w,
+                      double *segment_current, double *segment_power_w, int32_t *conductor_of_segment,
+                      size_t *n_conductors, int64_t *peak_filament, double *port_voltage,
+                      fh_power_balance *balance) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  if (!(std::isfinite(frequency) && frequency >= 0.0))  // check_frequency (solve.rs:992+)
+    return set_error(FH_E_INVALID_FREQUENCY,
+                     "frequency 0 is " + std::to_string(frequency) +
+                         " Hz; frequencies must be finite and non-negative",
+                     0, 0, frequency);
+  const size_t n = s->filaments.size(), m = s->basis.internal, p = s->basis.loops - m;
+  bool ok = n_excitation == p && (p == 0 || excitation);  // check_excitation (solve.rs:976-990)
+  for (size_t e = 0; ok && e < 2 * p; ++e) ok = std::isfinite(excitation[e]);
+  if (!ok)
+    return set_error(FH_E_INVALID_EXCITATION,
+                     "excitation must have one finite entry per port (" + std::to_string(p) + "), got " +
+                         std::to_string(n_excitation) + " entries",
+                     static_cast<int64_t>(p), static_cast<int64_t>(n_excitation));
+  using cd = std::complex<double>;
+  auto cmul = [](cd a, cd b) {
+    return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+  };
+  const double omega = 6.283185307179586 * frequency;
+  std::vector<cd> ex(p), internal(m), voltage(p);
+  for (size_t a = 0; a < p; ++a) ex[a] = cd(excitation[2 * a], excitation[2 * a + 1]);
+  if (s->iter) {
+    std::vector<double> in, v;
+    GmresOutcome o;
+    fh_status st = s->iter->currents(frequency, excitation, in, v, o);
+    if (st) return st;
+    for (size_t i = 0; i < m; ++i) internal[i] = cd(in[2 * i], in[2 * i + 1]);
+    for (size_t a = 0; a < p; ++a) voltage[a] = cd(v[2 * a], v[2 * a + 1]);
+  } else {
+    std::vector<double> X;
+    fh_status st = refined_x(s, &frequency, 1, X);
+    if (st) return st;
+    // internal = -x I_p
+    for (size_t i = 0; i < m; ++i) {
+      cd acc(0.0, 0.0);
+      for (size_t a = 0; a < p; ++a)
+        acc += cmul(cd(X[(2 * a) * m + i], X[(2 * a + 1) * m + i]), ex[a]);
+      internal[i] = -acc;
+    }
+    // V_p = Zpp I_p + Zep^T internal, from the port rows of the mesh blocks
+    const size_t loops = s->basis.loops;
+    std::vector<double> lrow(p * loops), rrow(p * loops, 0.0);
+    if (p)
+      FH_HIP_TRY(hipMemcpy(lrow.data(), s->d_lmesh.ptr + m * loops, lrow.size() * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < s->r_vals.size(); ++e)
+      if (static_cast<size_t>(s->r_rows[e]) >= m)
+        rrow[(static_cast<size_t>(s->r_rows[e]) - m) * loops + s->r_cols[e]] = s->r_vals[e];
+    const bool dc = frequency == 0.0;
+    auto z = [&](size_t a, size_t j) {  // Z[m + a][j], symmetric
+      return cd(rrow[a * loops + j], dc ? 0.0 : omega * lrow[a * loops + j]);
+    };
+    for (size_t a = 0; a < p; ++a) {
+      cd acc(0.0, 0.0);
+      for (size_t b = 0; b < p; ++b) acc += cmul(z(a, m + b), ex[b]);
+      cd tr(0.0, 0.0);
+      for (size_t i = 0; i < m; ++i) tr += cmul(z(a, i), internal[i]);
+      voltage[a] = acc + tr;
+    }
+  }
+  for (size_t i = 0; i < m; ++i)
+    if (!(std::isfinite(internal[i].real()) && std::isfinite(internal[i].imag())))
+      return set_error(FH_E_SINGULAR,
+                       "the mesh system is singular or ill-conditioned at " + std::to_string(frequency) + " Hz",
+                       0, 0, frequency);
+  for (size_t a = 0; a < p; ++a)
+    if (!(std::isfinite(voltage[a].real()) && std::isfinite(voltage[a].imag())))
+      return set_error(FH_E_SINGULAR,
+                       "the mesh system is singular or ill-conditioned at " + std::to_string(frequency) + " Hz",
+                       0, 0, frequency);
+  // branch = M^T [internal; excitation] (currents.rs:200-206)
+  std::vector<cd> branch(n, cd(0.0, 0.0));
+  const LoopBasis &B = s->basis;
+  for (size_t l = 0; l < B.loops; ++l) {
+    const cd x = l < m ? internal[l] : ex[l - m];
+    for (int64_t e = B.row_ptr[l]; e < B.row_ptr[l + 1]; ++e) branch[B.branch[e]] += x * B.sign[e];
+  }
+  // Layout (currents.rs:144-184): conductors by union of segment end nodes
+  const size_t nodes = s->prob_nodes.size() / 3, nseg = s->prob_segments.size();
+  std::vector<size_t> parent(nodes);
+  std::iota(parent.begin(), parent.end(), size_t(0));
+  auto find = [&](size_t x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (const fh_segment &d : s->prob_segments) {
+    const size_t a = find(d.a), b = find(d.b);
+    if (a != b) parent[a] = b;
+  }
+  std::vector<size_t> roots;
+  std::vector<int32_t> conductor(nseg);
+  for (size_t q = 0; q < nseg; ++q) {
+    const size_t r = find(s->prob_segments[q].a);
+    size_t c = std::find(roots.begin(), roots.end(), r) - roots.begin();
+    if (c == roots.size()) roots.push_back(r);
+    conductor[q] = static_cast<int32_t>(c);
+  }
+  std::vector<cd> seg_current(nseg, cd(0.0, 0.0));
+  std::vector<double> seg_power(nseg, 0.0), peak(roots.size(), -1.0);
+  std::vector<int64_t> peak_at(roots.size(), -1);
+  double dissipated = 0.0;
+  size_t f = 0;
+  for (size_t q = 0; q < nseg; ++q) {
+    const size_t per = static_cast<size_t>(s->prob_grids[q].nw) * s->prob_grids[q].nh;
+    for (size_t k = 0; k < per; ++k, ++f) {
+      const fh_filament &F = s->filaments[f];
+      const cd i = branch[f];
+      const double density = std::hypot(i.real(), i.imag()) / (F.width * F.height);
+      const double power = 0.5 * (i.real() * i.real() + i.imag() * i.imag()) * s->resistances[f];
+      dissipated += power;
+      seg_current[q] += i;
+      seg_power[q] += power;
+      const int32_t c = conductor[q];
+      if (peak_at[c] < 0 || density > peak[c]) {
+        peak[c] = density;
+        peak_at[c] = static_cast<int64_t>(f);
+      }
+      if (filament_current) {
+        filament_current[2 * f] = i.real();
+        filament_current[2 * f + 1] = i.imag();
+      }
+      if (current_density) current_density[f] = density;
+      if (power_w) power_w[f] = power;
+    }
+  }
+  double input = 0.0;
+  for (size_t a = 0; a < p; ++a) input += (std::conj(ex[a]) * voltage[a]).real();
+  input *= 0.5;
+  const double scale = std::max(dissipated, input);
+  for (size_t q = 0; q < nseg; ++q) {
+    if (segment_current) {
+      segment_current[2 * q] = seg_current[q].real();
+      segment_current[2 * q + 1] = seg_current[q].imag();
+    }
+    if (segment_power_w) segment_power_w[q] = seg_power[q];
+    if (conductor_of_segment) conductor_of_segment[q] = conductor[q];
+  }
+  if (n_conductors) *n_conductors = roots.size();
+  if (peak_filament)
+    for (size_t c = 0; c < roots.size(); ++c) peak_filament[c] = peak_at[c];
+  for (size_t a = 0; a < p && port_voltage; ++a) {
+    port_voltage[2 * a] = voltage[a].real();
+    port_voltage[2 * a + 1] = voltage[a].imag();
+  }
+  if (balance) {
+    balance->dissipated_w = dissipated;
+    balance->input_w = input;
+    balance->relative_residual = scale > 0.0 ? std::fabs(dissipated - input) / scale : 0.0;
   }
   return FH_OK;
 }

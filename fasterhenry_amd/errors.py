@@ -61,3 +61,8 @@ class PfftError(SolveError):
     """The pFFT operator cannot be built (pfft/mod.rs:236-251): code 1 is
     InvalidParameter, code 2 GridTooLarge (index0 = padded points, index1 =
     the limit)."""
+
+
+class InvalidExcitation(SolveError):
+    """The excitation does not have one finite entry per port
+    (SolveError::InvalidExcitation): index0 = ports, index1 = entries given."""

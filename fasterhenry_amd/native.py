@@ -30,7 +30,7 @@ SYMBOLS = [
     "fh_error_detail", "fh_version", "fh_iterative_default_params", "fh_system_create_iterative",
     "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
     "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
-    "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative",
+    "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative", "fh_currents",
 ]
 
 _ERRORS = {
@@ -38,6 +38,7 @@ _ERRORS = {
     4: errors.KernelError, 5: errors.MeshError, 6: errors.InvalidGrading,
     7: errors.InvalidFrequency, 8: errors.Singular, 9: errors.NotConverged,
     10: errors.SolveError, 11: errors.SolveError, 12: errors.HipError, 13: errors.PfftError,
+    14: errors.InvalidExcitation,
 }
 
 
@@ -61,6 +62,12 @@ class SweepInfo(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PowerBalance(C.Structure):
+    """fh_power_balance: PowerBalance (currents.rs:103-111)."""
+    _fields_ = [("dissipated_w", C.c_double), ("input_w", C.c_double),
+                ("relative_residual", C.c_double)]
 
 
 class ParamMap(C.Structure):
@@ -178,6 +185,9 @@ def lib() -> C.CDLL:
         L.fh_discretize_tangent.argtypes = [C.POINTER(Problem), vp, vp, vp, vp, sz]
         L.fh_grad.restype = st
         L.fh_grad.argtypes = [vp, C.POINTER(ParamMap), vp, sz, vp, vp, vp]
+        L.fh_currents.restype = st
+        L.fh_currents.argtypes = [vp, C.c_double, vp, sz, vp, vp, vp, vp, vp, vp, C.POINTER(sz), vp, vp,
+                                  C.POINTER(PowerBalance)]
         L.fh_partial_inductance_derivative.restype = st
         L.fh_partial_inductance_derivative.argtypes = [vp, sz, vp, sz, vp, sz, vp, vp]
         L.fh_solve_batch.restype = st

@@ -46,8 +46,10 @@ typedef enum fh_status {
   FH_E_UNSUPPORTED = 10,     /* a path this build does not provide */
   FH_E_NO_MEMORY = 11,       /* host or device allocation failed */
   FH_E_HIP = 12,             /* HIP runtime error */
-  FH_E_PFFT = 13             /* PfftError (pfft/mod.rs:236-251): code 1 = InvalidParameter,
+  FH_E_PFFT = 13,            /* PfftError (pfft/mod.rs:236-251): code 1 = InvalidParameter,
                                 2 = GridTooLarge (index0 = padded points, index1 = limit) */
+  FH_E_INVALID_EXCITATION = 14 /* SolveError::InvalidExcitation: index0 = ports expected,
+                                  index1 = entries given */
 } fh_status;
 
 /* SolverChoice (solve.rs:169-189) and Solver (solve.rs:143-158). */
@@ -270,6 +272,34 @@ typedef struct fh_param_map {
   const double *d_widths;
   const double *d_heights;
 } fh_param_map;
+
+/* PowerBalance (currents.rs:103-111), watts, peak-phasor convention. */
+typedef struct fh_power_balance {
+  double dissipated_w;      /* sum of 1/2 |I|^2 R over filaments */
+  double input_w;           /* 1/2 Re(I_p^H V_p) */
+  double relative_residual; /* |dissipated - input| / max(dissipated, input), 0 if both 0 */
+} fh_power_balance;
+
+/* MeshSystem::currents / IterativeSystem::currents (solve.rs:835-868,
+ * iterative.rs:329-380, solve_currents solve.rs:1110-1149,
+ * assemble_solution currents.rs:190-274): for one frequency and the port
+ * current phasors `excitation` (n_excitation = p complex {re, im}), the
+ * internal loop currents -Zee^-1 Zep I_p mapped to filaments.  Outputs, each
+ * may be NULL: filament currents (n complex), |J| = |I| / (w h) (n), power
+ * 1/2 |I|^2 R (n); per segment the summed current (complex) and power, and
+ * its conductor (connected component, numbered by first segment); the
+ * number of conductors and, per conductor, the filament of largest |J|
+ * (capacity n_segments; the first of equals); the port voltages
+ * Zpp I_p + Zep^T I_e (p complex); the power balance.  The dense form
+ * solves with the refined factorization refined to working accuracy; the
+ * iterative form with GMRES.  Errors: FH_E_INVALID_FREQUENCY,
+ * FH_E_INVALID_EXCITATION (wrong length or a non-finite entry),
+ * FH_E_SINGULAR, FH_E_NOT_CONVERGED. */
+fh_status fh_currents(fh_system *system, double frequency_hz, const double *excitation,
+                      size_t n_excitation, double *filament_current, double *current_density,
+                      double *power_w, double *segment_current, double *segment_power_w,
+                      int32_t *conductor_of_segment, size_t *n_conductors, int64_t *peak_filament,
+                      double *port_voltage, fh_power_balance *balance);
 
 /* Gradients of the port impedance.  No counterpart in the reference (it has
  * no derivatives); the north star's adjoint: for each frequency k, parameter

@@ -320,6 +320,82 @@ def solve(packed, freqs, threads: int = 0) -> np.ndarray:
     return System(packed, threads).sweep(freqs)
 
 
+def currents(packed, frequency: float, excitation, system: "System" = None) -> dict:
+    """solve_currents (solve.rs:1110-1149) and assemble_solution
+    (currents.rs:190-274) on the oracle's mesh blocks: internal loop
+    currents -Zee^-1 (Zep I_p) by the oracle's LU, port voltages
+    Zpp I_p + Zep^T I_e, branch currents M^T [I_e; I_p], then |J|, power,
+    segment sums, conductors (Layout::new, currents.rs:144-184) and the
+    power balance.  Returns a dict of numpy arrays and scalars."""
+    system = system or System(packed)
+    ex = np.asarray(excitation, dtype=np.complex128).reshape(-1)
+    b = system.blocks()
+    m, p = system.internal, system.ports
+    w = 2.0 * np.pi * frequency
+    ep = b["r_ep"] + 1j * w * b["l_ep"]
+    pp = b["r_pp"] + 1j * w * b["l_pp"]
+    if m:
+        ee = b["r_ee"] + 1j * w * b["l_ee"]
+        internal = -lu_solve(ee, ep @ ex)
+    else:
+        internal = np.zeros(0, dtype=np.complex128)
+    voltage = pp @ ex + ep.T @ internal
+    row_ptr, branch, sign, _ = mesh_build(packed)
+    f = system.filament_array()
+    n = len(f)
+    loops = np.concatenate([internal, ex])
+    cur = np.zeros(n, dtype=np.complex128)
+    for l in range(len(loops)):
+        for e in range(row_ptr[l], row_ptr[l + 1]):
+            cur[branch[e]] += loops[l] * sign[e]
+    width, height = np.asarray(f["width"]), np.asarray(f["height"])
+    res = np.asarray(f["length"]) / (np.asarray(f["sigma"]) * (width * height))
+    density = np.abs(cur) / (width * height)
+    power = 0.5 * np.abs(cur) ** 2 * res
+    nodes = len(packed.nodes)
+    parent = list(range(nodes))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    segs = packed.segments
+    for q in range(len(segs)):
+        a, c = find(int(segs[q]["a"])), find(int(segs[q]["b"]))
+        if a != c:
+            parent[a] = c
+    roots, conductor = [], []
+    for q in range(len(segs)):
+        r = find(int(segs[q]["a"]))
+        if r not in roots:
+            roots.append(r)
+        conductor.append(roots.index(r))
+    per = packed.grids["nw"].astype(np.int64) * packed.grids["nh"].astype(np.int64)
+    seg_of = np.repeat(np.arange(len(segs)), per)
+    seg_current = np.zeros(len(segs), dtype=np.complex128)
+    seg_power = np.zeros(len(segs))
+    peak = {}
+    dissipated = 0.0
+    for k in range(n):
+        q = seg_of[k]
+        dissipated += power[k]
+        seg_current[q] += cur[k]
+        seg_power[q] += power[k]
+        c = conductor[q]
+        if c not in peak or density[k] > density[peak[c]]:
+            peak[c] = k
+    inp = 0.5 * float(np.sum((np.conj(ex) * voltage).real))
+    scale = max(dissipated, inp)
+    return dict(current=cur, density=density, power=power, segment_current=seg_current,
+                segment_power=seg_power, conductor=np.array(conductor),
+                peaks=[peak[c] for c in range(len(roots))], voltage=voltage,
+                location=(np.asarray(f["start"]) + np.asarray(f["end"])) * 0.5,
+                dissipated_w=dissipated, input_w=inp,
+                relative_residual=abs(dissipated - inp) / scale if scale > 0 else 0.0)
+
+
 # ---------------------------------------------------------------------------
 # pfft/*.rs, gmres.rs, iterative.rs
 
