@@ -15,4 +15,5 @@ from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # no
                   unit_excitation,
                   Timing, device_math, discretize_geometry, discretize_packed, discretize_tangent,
                   gmres_dense, iterative_params, partial_inductance_derivative,
-                  partial_inductance_matrix, pfft_params, solve, solve_batch, version)
+                  partial_inductance_matrix, pfft_params, solve, solve_batch, spice_subckt,
+                  subckt_name_from_path, version, zc_mat)

@@ -102,6 +102,64 @@ class SweepResult:
         }
         return json.dumps(doc)
 
+    def to_zc_mat(self) -> bytes:
+        """MAT v4 bytes of ``freqs`` and ``Zc_1 .. Zc_K`` (fasterhenry-cli/src/mat.rs:31-95)."""
+        return zc_mat(self.frequencies_hz, self.impedance_ohm)
+
+    def write_zc_mat(self, path: str) -> None:
+        with open(path, "wb") as fh:
+            fh.write(self.to_zc_mat())
+
+    def to_spice(self, index: int, name: str) -> str:
+        """The SPICE subcircuit stamp at frequency ``index``
+        (fasterhenry-cli/src/spice.rs:64-126)."""
+        return spice_subckt(self.frequencies_hz, self.impedance_ohm, index, name)
+
+
+def _z_array(frequencies, z) -> tuple:
+    f = np.ascontiguousarray(frequencies, dtype=np.float64).reshape(-1)
+    zz = np.ascontiguousarray(z, dtype=np.complex128)
+    if zz.ndim != 3 or zz.shape[0] != len(f) or zz.shape[1] != zz.shape[2]:
+        raise InvalidArgument(f"z must be (K, p, p) for K = {len(f)} frequencies, got {zz.shape}")
+    return f, zz
+
+
+def zc_mat(frequencies, z) -> bytes:
+    """write_zc_mat (fasterhenry-cli/src/mat.rs:31-95) of a (K, p, p) sweep."""
+    f, zz = _z_array(frequencies, z)
+    L = native.lib()
+    size = C.c_size_t()
+    native.check(L.fh_zc_mat(native.ptr(f), len(f), native.ptr(zz), zz.shape[1], None, 0, C.byref(size)))
+    out = (C.c_uint8 * max(size.value, 1))()
+    native.check(L.fh_zc_mat(native.ptr(f), len(f), native.ptr(zz), zz.shape[1], out, size.value,
+                             C.byref(size)))
+    return bytes(out[:size.value])
+
+
+def spice_subckt(frequencies, z, index: int, name: str) -> str:
+    """write_spice_subckt (fasterhenry-cli/src/spice.rs:64-126)."""
+    f, zz = _z_array(frequencies, z)
+    L = native.lib()
+    length = C.c_size_t()
+    nm = name.encode()
+    native.check(L.fh_spice_subckt(native.ptr(f), len(f), native.ptr(zz), zz.shape[1], index, nm, None, 0,
+                                   C.byref(length)))
+    buf = C.create_string_buffer(length.value + 1)
+    native.check(L.fh_spice_subckt(native.ptr(f), len(f), native.ptr(zz), zz.shape[1], index, nm, buf,
+                                   length.value + 1, C.byref(length)))
+    return buf.value.decode()
+
+
+def subckt_name_from_path(path: str) -> str:
+    """subckt_name_from_path (fasterhenry-cli/src/spice.rs:37-60)."""
+    L = native.lib()
+    length = C.c_size_t()
+    raw = path.encode("utf-8", "surrogateescape")
+    native.check(L.fh_subckt_name_from_path(raw, None, 0, C.byref(length)))
+    buf = C.create_string_buffer(length.value + 1)
+    native.check(L.fh_subckt_name_from_path(raw, buf, length.value + 1, C.byref(length)))
+    return buf.value.decode()
+
 
 @dataclass
 class FilamentCurrent:

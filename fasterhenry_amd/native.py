@@ -31,6 +31,7 @@ SYMBOLS = [
     "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
     "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
     "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative", "fh_currents",
+    "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path",
 ]
 
 _ERRORS = {
@@ -185,6 +186,12 @@ def lib() -> C.CDLL:
         L.fh_discretize_tangent.argtypes = [C.POINTER(Problem), vp, vp, vp, vp, sz]
         L.fh_grad.restype = st
         L.fh_grad.argtypes = [vp, C.POINTER(ParamMap), vp, sz, vp, vp, vp]
+        L.fh_zc_mat.restype = st
+        L.fh_zc_mat.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz)]
+        L.fh_spice_subckt.restype = st
+        L.fh_spice_subckt.argtypes = [vp, sz, vp, sz, sz, C.c_char_p, C.c_char_p, sz, C.POINTER(sz)]
+        L.fh_subckt_name_from_path.restype = st
+        L.fh_subckt_name_from_path.argtypes = [C.c_char_p, C.c_char_p, sz, C.POINTER(sz)]
         L.fh_currents.restype = st
         L.fh_currents.argtypes = [vp, C.c_double, vp, sz, vp, vp, vp, vp, vp, vp, C.POINTER(sz), vp, vp,
                                   C.POINTER(PowerBalance)]

@@ -301,6 +301,20 @@ fh_status fh_currents(fh_system *system, double frequency_hz, const double *exci
                       int32_t *conductor_of_segment, size_t *n_conductors, int64_t *peak_filament,
                       double *port_voltage, fh_power_balance *balance);
 
+/* Writers of a sweep (K frequencies, z K x p x p {re, im} row-major, as
+ * fh_sweep returns it).  fh_zc_mat: write_zc_mat
+ * (fasterhenry-cli/src/mat.rs:31-95), the MAT v4 bytes of `freqs` (1 x K)
+ * and `Zc_1 .. Zc_K` (complex p x p); *size receives the byte count, out
+ * (may be NULL) the bytes.  fh_spice_subckt: write_spice_subckt
+ * (fasterhenry-cli/src/spice.rs:64-126) at frequency `index`, the text
+ * with its terminating NUL in out (may be NULL), *length its length.
+ * fh_subckt_name_from_path: subckt_name_from_path (spice.rs:37-60). */
+fh_status fh_zc_mat(const double *freqs, size_t K, const double *z, size_t p, uint8_t *out,
+                    size_t capacity, size_t *size);
+fh_status fh_spice_subckt(const double *freqs, size_t K, const double *z, size_t p, size_t index,
+                          const char *name, char *out, size_t capacity, size_t *length);
+fh_status fh_subckt_name_from_path(const char *path, char *out, size_t capacity, size_t *length);
+
 /* Gradients of the port impedance.  No counterpart in the reference (it has
  * no derivatives); the north star's adjoint: for each frequency k, parameter
  * q and port pair (a, b), dR_dp = Re dZ_ab/dp_q and dL_dp = Im dZ_ab/dp_q / w
