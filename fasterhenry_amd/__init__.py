@@ -8,6 +8,8 @@ hand-written HIP kernels in libfasterhenry_gpu.so through its C ABI
 solve loads the native library and raises if it or the device is missing.
 """
 from .errors import *  # noqa: F401,F403
+from .coupling import (DEFAULT_GROUP, Coupling, CouplingError, NonTransitiveCoupling,  # noqa: F401
+                       PairMask, SegmentCountMismatch, TruncationWarning, UnknownGroup)
 from .model import *  # noqa: F401,F403
 from .model import DENSE_PATH_MAX_FILAMENTS, MU0  # noqa: F401
 from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # noqa: F401
@@ -15,5 +17,6 @@ from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # no
                   unit_excitation,
                   Timing, device_math, discretize_geometry, discretize_packed, discretize_tangent,
                   gmres_dense, iterative_params, partial_inductance_derivative,
-                  partial_inductance_matrix, pfft_params, solve, solve_batch, spice_subckt,
+                  partial_inductance_matrix, partial_inductance_matrix_masked, pfft_params, solve,
+                  solve_batch, spice_subckt,
                   subckt_name_from_path, version, zc_mat)

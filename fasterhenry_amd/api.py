@@ -25,6 +25,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import native
+from .coupling import Coupling, PairMask
 from .errors import InvalidArgument, SolveError
 from .model import (DENSE_PATH_MAX_FILAMENTS, FILAMENT_DTYPE, Discretization, Geometry,
                     PackedProblem, Port, Solver, SolverChoice, pack)
@@ -290,12 +291,38 @@ class MeshSystem:
         self.ports = list(ports)
         self.precision = precision
         self._counts = _counts(handle)
+        self.truncation_warnings: List = []
 
     @classmethod
     def assemble(cls, geometry: Geometry, ports: Sequence[Port], discretization: Discretization,
                  precision: str = "fp64", solver: SolverChoice = SolverChoice.DENSE) -> "MeshSystem":
         packed = pack(geometry, ports, discretization)
         return cls.assemble_packed(packed, ports, precision, solver)
+
+    @classmethod
+    def assemble_with_coupling(cls, geometry: Geometry, ports: Sequence[Port],
+                               discretization: Discretization, coupling: Coupling,
+                               precision: str = "fp64",
+                               solver: SolverChoice = SolverChoice.DENSE) -> "MeshSystem":
+        """MeshSystem::assemble_with_coupling (solve.rs:703-766): the mutual
+        inductance of uncoupled groups is never evaluated.  The coupling is
+        validated (and its warnings computed) before anything is assembled."""
+        warnings = coupling.truncation_warnings(geometry)
+        packed = pack(geometry, ports, discretization)
+        if coupling.is_all_pairs():
+            system = cls.assemble_packed(packed, ports, precision, solver)
+        else:
+            group, groups, table = coupling.resolved(geometry.segment_count())
+            cm = native.CouplingMask(native.ptr(group) if len(group) else None, len(group), groups,
+                                     native.ptr(table))
+            L = native.lib()
+            h = C.c_void_p()
+            prob = native.problem_struct(packed)
+            native.check(L.fh_system_create_coupled(C.byref(prob), solver.value, _precision_code(precision),
+                                                    C.byref(cm), C.byref(h)))
+            system = cls(h, packed, ports, precision)
+        system.truncation_warnings = warnings
+        return system
 
     @classmethod
     def assemble_packed(cls, packed: PackedProblem, ports: Sequence[Port], precision: str = "fp64",
@@ -697,6 +724,22 @@ def partial_inductance_derivative(filaments: np.ndarray, tangents: np.ndarray, p
         native.ptr(f), len(f), native.ptr(t), t.shape[0], native.ptr(pr), len(pr), native.ptr(v),
         native.ptr(d)))
     return v, d
+
+
+def partial_inductance_matrix_masked(filaments: np.ndarray, mask: PairMask, with_methods: bool = False):
+    """partial_inductance_matrix_masked (inductance/batch.rs:305-312): pairs
+    the mask rejects are exactly 0 and never evaluated (method -1)."""
+    f = np.ascontiguousarray(filaments, dtype=FILAMENT_DTYPE)
+    n = len(f)
+    group = np.ascontiguousarray(mask.group, dtype=np.int32)
+    table = np.ascontiguousarray(mask.coupled, dtype=np.uint8)
+    pm = native.PairMask(native.ptr(group) if len(group) else None, len(group), mask.groups,
+                         native.ptr(table) if table.size else None)
+    L = np.zeros((n, n))
+    methods = np.zeros((n, n), dtype=np.int8) if with_methods else None
+    native.check(native.lib().fh_partial_inductance_masked(
+        native.ptr(f), n, C.byref(pm), native.ptr(L), native.ptr(methods) if with_methods else None))
+    return (L, methods) if with_methods else L
 
 
 def partial_inductance_matrix(filaments: np.ndarray, with_methods: bool = False):

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -49,7 +50,7 @@ constexpr int kInlineEvals = 1024;   // point-quadrature evaluations done in the
 constexpr int kStackCapacity = 208;  // > the reference's 200-level guard
 
 enum Method : int8_t { kOrthogonal = 0, kBarClosedForm = 1, kAlignedQuadrature = 2,
-                       kPointQuadrature = 3, kSampledFilaments = 4 };
+                       kPointQuadrature = 3, kSampledFilaments = 4, kTruncated = -1 };
 enum Kind : uint32_t { kHeavyPoint = 1u, kAligned = 2u, kSampled = 3u };
 
 __constant__ double c_gx[kMaxRule + 1][kMaxRule];
@@ -760,12 +761,31 @@ __device__ __forceinline__ void tile_coords(long long t, int nt, int &ti, int &t
   tj = r + static_cast<int>(t - start(r));
 }
 
+// PairMask::keeps (inductance/batch.rs:76-85): pairs of filaments whose
+// groups are not coupled are never evaluated.  A filament without a group
+// entry, or a table index out of range, keeps the pair.
+struct MaskArgs {
+  const int32_t *group;  // null: every pair kept
+  int n, groups;
+  const uint8_t *table;  // groups x groups
+};
+
+__device__ __forceinline__ bool mask_keeps(const MaskArgs &m, int i, int j) {
+  if (!m.group || i >= m.n || j >= m.n) return true;
+  const int a = m.group[i], b = m.group[j];
+  if (a < 0 || b < 0) return true;
+  const long long flat = static_cast<long long>(a) * m.groups + b;
+  if (flat >= static_cast<long long>(m.groups) * m.groups) return true;
+  return m.table[flat] != 0;
+}
+
 __global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__restrict__ F, int n,
                                                         int nt, double *__restrict__ L,
                                                         int8_t *__restrict__ methods,
                                                         uint2 *__restrict__ queue,
                                                         unsigned int *__restrict__ queue_count,
-                                                        unsigned long long *__restrict__ err) {
+                                                        unsigned long long *__restrict__ err,
+                                                        MaskArgs mask) {
   __shared__ fh_filament rows[kTile], cols[kTile];
   __shared__ double vals[kTile][kTile + 1];
   __shared__ int8_t meth[kTile][kTile + 1];
@@ -789,7 +809,10 @@ __global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__res
   double value = 0.0;
   int8_t method = kOrthogonal;
   bool finished = false;
-  if (i < n && j < n && i <= j) {
+  if (i < n && j < n && i <= j && !mask_keeps(mask, i, j)) {
+    finished = true;  // truncated: exactly 0, never evaluated
+    method = kTruncated;
+  } else if (i < n && j < n && i <= j) {
     PairPlan p;
     plan_pair(rows[r], cols[c], p);
     if (p.orthogonal) {
@@ -1226,7 +1249,7 @@ fh_status memo_aligned(const fh_filament *d_F, int n, const uint2 *queue, const 
 }  // namespace
 
 fh_status fill_inductance(const fh_filament *d_F, size_t n, double *d_L, int8_t *d_methods,
-                          hipStream_t stream) {
+                          hipStream_t stream, const DevicePairMask *pair_mask) {
   if (n == 0) return FH_OK;
   if (n >= (1u << 30)) return set_error(FH_E_INVALID_ARGUMENT, "too many filaments");
   fh_status st = upload_tables();
@@ -1243,8 +1266,12 @@ fh_status fill_inductance(const fh_filament *d_F, size_t n, double *d_L, int8_t 
   FH_HIP_TRY(hipMemsetAsync(qcount.ptr, 0, sizeof(unsigned int), stream));
   FH_HIP_TRY(hipMemsetAsync(err.ptr, 0xff, sizeof(unsigned long long), stream));
   if (tiles > 0x7fffffffLL) return set_error(FH_E_INVALID_ARGUMENT, "too many tiles");
+  MaskArgs mask{nullptr, 0, 0, nullptr};
+  if (pair_mask && pair_mask->group && pair_mask->table)
+    mask = MaskArgs{pair_mask->group, static_cast<int>(std::min<size_t>(pair_mask->n, n)),
+                    static_cast<int>(pair_mask->groups), pair_mask->table};
   fill_tile_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
-      d_F, static_cast<int>(n), nt, d_L, d_methods, queue.ptr, qcount.ptr, err.ptr);
+      d_F, static_cast<int>(n), nt, d_L, d_methods, queue.ptr, qcount.ptr, err.ptr, mask);
   FH_HIP_TRY(hipGetLastError());
   int dev = 0, cus = 256;
   FH_HIP_TRY(hipGetDevice(&dev));

@@ -90,11 +90,20 @@ fh_status discretize_tangent(const fh_problem &p, const double *dn, const double
 // Pair kernel tables in device memory (fh_fill.hip).
 fh_status fill_tables_ready();
 
+// A pair mask in device memory (PairMask, inductance/batch.rs:37-110): the
+// group of each of the first n filaments and a groups x groups keep table.
+struct DevicePairMask {
+  const int32_t *group = nullptr;
+  size_t n = 0, groups = 0;
+  const uint8_t *table = nullptr;
+};
+
 // GPU fill (fh_fill.hip): L (n x n row-major, device) of device filaments.
-// methods (device, may be null) receives each entry's method.  On a kernel
-// error returns FH_E_KERNEL with the lowest failing pair.
+// methods (device, may be null) receives each entry's method (-1 for a pair
+// the mask truncates, whose entry is exactly 0).  On a kernel error returns
+// FH_E_KERNEL with the lowest failing pair.
 fh_status fill_inductance(const fh_filament *d_filaments, size_t n, double *d_L,
-                          int8_t *d_methods, hipStream_t stream);
+                          int8_t *d_methods, hipStream_t stream, const DevicePairMask *mask = nullptr);
 
 // GPU congruence (fh_congruence.hip): l_mesh = M L M^T (loops x loops,
 // row-major, device) following solve.rs:1172-1200's summation order.

@@ -227,6 +227,24 @@ size_t sweep_budget_bytes() {
 }
 
 
+// A host pair mask (group per filament, symmetric keep table) to the device.
+fh_status upload_mask(const fh_pair_mask *mask, DeviceArray<int32_t> &d_group, DeviceArray<uint8_t> &d_table,
+                      DevicePairMask &out, hipStream_t stream) {
+  out = DevicePairMask{};
+  if (!mask || mask->n == 0) return FH_OK;
+  fh_status st;
+  const size_t tab = mask->groups * mask->groups;
+  if ((st = d_group.allocate(mask->n)) || (st = d_table.allocate(std::max<size_t>(tab, 1)))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(d_group.ptr, mask->group, mask->n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  if (tab)
+    FH_HIP_TRY(hipMemcpyAsync(d_table.ptr, mask->coupled, tab, hipMemcpyHostToDevice, stream));
+  out.group = d_group.ptr;
+  out.n = mask->n;
+  out.groups = mask->groups;
+  out.table = d_table.ptr;
+  return FH_OK;
+}
+
 // x = Zee^-1 Zep (2 K p vectors of m: real then imaginary part of column a
 // at frequency k, vector 2 (k p + a)) by the refined solve, refined until
 // the correction is at the rounding level (tight_x); first failure in list
@@ -340,6 +358,11 @@ fh_status fh_discretize_tangent(const fh_problem *problem, const double *d_nodes
 
 fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
                            fh_precision precision, fh_system **out) {
+  return fh_system_create_coupled(problem, choice, precision, nullptr, out);
+}
+
+fh_status fh_system_create_coupled(const fh_problem *problem, fh_solver_choice choice, fh_precision precision,
+                                   const fh_coupling *coupling, fh_system **out) {
   clear_error();
   if (!problem || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
@@ -351,7 +374,18 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
     nfil += static_cast<size_t>(problem->grids[s].nw) * problem->grids[s].nh;
   if (choice != FH_AUTO && choice != FH_DENSE && choice != FH_ITERATIVE)
     return set_error(FH_E_INVALID_ARGUMENT, "unknown solver choice");
-  if (fh_resolve(choice, nfil) != FH_SOLVER_DENSE) {
+  const bool truncating = coupling && coupling->coupled;
+  if (truncating) {
+    if (coupling->n_segments != problem->n_segments || (problem->n_segments && !coupling->segment_group))
+      return set_error(FH_E_INVALID_ARGUMENT, "the coupling must give one group per segment");
+    // fasterhenry-cli/src/lib.rs:89-107: the matrix-free operator couples
+    // every pair, so a truncation stays dense
+    if (choice == FH_ITERATIVE)
+      return set_error(FH_E_UNSUPPORTED,
+                       "the iterative solver cannot honour a coupling truncation: the matrix-free operator "
+                       "couples every pair of filaments");
+  }
+  if (!truncating && fh_resolve(choice, nfil) != FH_SOLVER_DENSE) {
     fh_iterative_params params;
     fh_iterative_default_params(&params);
     fh_status st = fh_system_create_iterative(problem, &params, out);
@@ -381,8 +415,21 @@ fh_status fh_system_create(const fh_problem *problem, fh_solver_choice choice,
     FH_HIP_TRY(hipMemcpyAsync(s->d_filaments.ptr, s->filaments.data(), n * sizeof(fh_filament),
                               hipMemcpyHostToDevice, s->stream));
   }
+  // PairMask from the segment groups (Coupling::pair_mask, coupling.rs:259-272)
+  std::vector<int32_t> fgroup;
+  fh_pair_mask hmask{nullptr, 0, 0, nullptr};
+  DeviceArray<int32_t> d_g;
+  DeviceArray<uint8_t> d_t;
+  DevicePairMask dmask;
+  if (truncating) {
+    fgroup.reserve(n);
+    for (size_t q = 0; q < problem->n_segments; ++q) fgroup.insert(fgroup.end(), per[q], coupling->segment_group[q]);
+    hmask = fh_pair_mask{fgroup.data(), fgroup.size(), coupling->groups, coupling->coupled};
+    if ((st = upload_mask(&hmask, d_g, d_t, dmask, s->stream))) return st;
+  }
   const auto ti = Clock::now();
-  if ((st = fill_inductance(s->d_filaments.ptr, n, s->d_L.ptr, nullptr, s->stream))) return st;
+  if ((st = fill_inductance(s->d_filaments.ptr, n, s->d_L.ptr, nullptr, s->stream, truncating ? &dmask : nullptr)))
+    return st;
   s->inductance_s = seconds_since(ti);
   const size_t loops = s->basis.loops;
   if ((st = s->d_lmesh.allocate(loops * loops))) return st;
@@ -846,8 +893,15 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
 
 fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *L_out,
                                 int8_t *methods) {
+  return fh_partial_inductance_masked(filaments, n, nullptr, L_out, methods);
+}
+
+fh_status fh_partial_inductance_masked(const fh_filament *filaments, size_t n, const fh_pair_mask *mask,
+                                       double *L_out, int8_t *methods) {
   clear_error();
   if ((n && (!filaments || !L_out))) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (mask && mask->n && (!mask->group || (mask->groups && !mask->coupled)))
+    return set_error(FH_E_INVALID_ARGUMENT, "null mask array");
   if (n == 0) return FH_OK;
   fh_status st = check_device();
   if (st) return st;
@@ -860,10 +914,15 @@ fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *
   DeviceArray<fh_filament> d_f;
   DeviceArray<double> d_L;
   DeviceArray<int8_t> d_m;
+  DeviceArray<int32_t> d_g;
+  DeviceArray<uint8_t> d_t;
   if ((st = d_f.allocate(n)) || (st = d_L.allocate(n * n))) return st;
   if (methods && (st = d_m.allocate(n * n))) return st;
   FH_HIP_TRY(hipMemcpyAsync(d_f.ptr, filaments, n * sizeof(fh_filament), hipMemcpyHostToDevice, stream));
-  if ((st = fill_inductance(d_f.ptr, n, d_L.ptr, methods ? d_m.ptr : nullptr, stream))) return st;
+  DevicePairMask dm;
+  if ((st = upload_mask(mask, d_g, d_t, dm, stream))) return st;
+  if ((st = fill_inductance(d_f.ptr, n, d_L.ptr, methods ? d_m.ptr : nullptr, stream, mask ? &dm : nullptr)))
+    return st;
   FH_HIP_TRY(hipMemcpyAsync(L_out, d_L.ptr, n * n * sizeof(double), hipMemcpyDeviceToHost, stream));
   if (methods)
     FH_HIP_TRY(hipMemcpyAsync(methods, d_m.ptr, n * n, hipMemcpyDeviceToHost, stream));

@@ -31,7 +31,8 @@ SYMBOLS = [
     "fh_system_operator_stats", "fh_pfft_default_params", "fh_pfft_create", "fh_pfft_apply",
     "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
     "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative", "fh_currents",
-    "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path",
+    "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path", "fh_system_create_coupled",
+    "fh_partial_inductance_masked",
 ]
 
 _ERRORS = {
@@ -63,6 +64,18 @@ class SweepInfo(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PairMask(C.Structure):
+    """fh_pair_mask: PairMask (inductance/batch.rs:37-110)."""
+    _fields_ = [("group", C.c_void_p), ("n", C.c_size_t), ("groups", C.c_size_t),
+                ("coupled", C.c_void_p)]
+
+
+class CouplingMask(C.Structure):
+    """fh_coupling: a Coupling resolved against the segments."""
+    _fields_ = [("segment_group", C.c_void_p), ("n_segments", C.c_size_t), ("groups", C.c_size_t),
+                ("coupled", C.c_void_p)]
 
 
 class PowerBalance(C.Structure):
@@ -186,6 +199,11 @@ def lib() -> C.CDLL:
         L.fh_discretize_tangent.argtypes = [C.POINTER(Problem), vp, vp, vp, vp, sz]
         L.fh_grad.restype = st
         L.fh_grad.argtypes = [vp, C.POINTER(ParamMap), vp, sz, vp, vp, vp]
+        L.fh_system_create_coupled.restype = st
+        L.fh_system_create_coupled.argtypes = [C.POINTER(Problem), C.c_int, C.c_int,
+                                               C.POINTER(CouplingMask), C.POINTER(vp)]
+        L.fh_partial_inductance_masked.restype = st
+        L.fh_partial_inductance_masked.argtypes = [vp, sz, C.POINTER(PairMask), vp, vp]
         L.fh_zc_mat.restype = st
         L.fh_zc_mat.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz)]
         L.fh_spice_subckt.restype = st

@@ -351,6 +351,45 @@ fh_status fh_partial_inductance_derivative(const fh_filament *filaments, size_t 
                                            const uint32_t *pairs, size_t count, double *value_out,
                                            double *deriv_out);
 
+/* PairMask (inductance/batch.rs:37-110): the group of each of n filaments
+ * and a groups x groups keep table (row-major; the caller symmetrizes it and
+ * sets the diagonal).  A pair is evaluated when either filament has no
+ * entry (index >= n) or a negative group, when the table index
+ * a * groups + b is past the table, or when the entry is nonzero; otherwise
+ * its L entry is exactly 0 and no kernel runs for it. */
+typedef struct fh_pair_mask {
+  const int32_t *group;
+  size_t n;
+  size_t groups;
+  const uint8_t *coupled;
+} fh_pair_mask;
+
+/* A coupling resolved against the segments (Coupling, coupling.rs:173-353):
+ * the group index of every segment and the groups x groups keep table.
+ * coupled = NULL is Coupling::all_pairs. */
+typedef struct fh_coupling {
+  const int32_t *segment_group;
+  size_t n_segments;
+  size_t groups;
+  const uint8_t *coupled;
+} fh_coupling;
+
+/* MeshSystem::assemble_with_coupling (solve.rs:703-766): as
+ * fh_system_create, with the partial inductances of uncoupled groups
+ * truncated (never evaluated).  A truncating coupling stays dense under
+ * FH_AUTO and returns FH_E_UNSUPPORTED under FH_ITERATIVE
+ * (fasterhenry-cli/src/lib.rs:89-107).  The coupling's names, validation and
+ * truncation warnings are resolved by the caller (the Python mirror's
+ * Coupling). */
+fh_status fh_system_create_coupled(const fh_problem *problem, fh_solver_choice choice,
+                                   fh_precision precision, const fh_coupling *coupling,
+                                   fh_system **out);
+
+/* partial_inductance_matrix_masked (inductance/batch.rs:305-312): mask may
+ * be NULL (every pair); methods (may be NULL) holds -1 for truncated pairs. */
+fh_status fh_partial_inductance_masked(const fh_filament *filaments, size_t n,
+                                       const fh_pair_mask *mask, double *L_out, int8_t *methods);
+
 /* Kernel level.  partial_inductance_matrix (inductance/batch.rs:246-253):
  * L_out (n x n, row-major, host) for host filaments.  methods (may be NULL)
  * receives the Method of inductance/mod.rs:130-152 per entry (0 orthogonal,
