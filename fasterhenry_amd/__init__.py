@@ -20,3 +20,4 @@ from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # no
                   partial_inductance_matrix, partial_inductance_matrix_masked, pfft_params, solve,
                   solve_batch, spice_subckt,
                   subckt_name_from_path, version, zc_mat)
+from . import deck  # noqa: E402,F401

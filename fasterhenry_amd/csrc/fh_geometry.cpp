@@ -158,6 +158,13 @@ void make(fh_filament &f, V3 start, V3 end, double length, V3 l, V3 w, V3 h, dou
 
 }  // namespace
 
+int segment_frame_code(const double *a, const double *b, const fh_segment &s) {
+  V3 l, w, h;
+  return segment_frame(load(a), load(b), s, l, w, h);
+}
+
+double rust_powi(double a, int b) { return powi(a, b); }
+
 fh_status validate_geometry(const fh_problem &p) {
   if ((p.n_nodes && !p.nodes) || (p.n_segments && (!p.segments || !p.grids)) ||
       (p.n_ports && !p.ports))

@@ -82,6 +82,12 @@ fh_status validate_geometry(const fh_problem &p);
 fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out);
 fh_status build_loop_basis(const fh_problem &p, const std::vector<size_t> &per_segment,
                            LoopBasis &out);
+// geometry.rs:223-276 frame check of one segment between points a and b: 0,
+// or the SegmentError sub-code (1 non-finite endpoint, 2 width/height, 3
+// conductivity, 4 zero length, 5 width direction).
+int segment_frame_code(const double *a, const double *b, const fh_segment &s);
+// Rust's f64::powi arithmetic (compiler-rt __powidf2).
+double rust_powi(double a, int b);
 // d(filaments) along a direction of the geometry (node moves dn, width and
 // height changes dw, dh; null = 0), counts and frames' rules held fixed.
 fh_status discretize_tangent(const fh_problem &p, const double *dn, const double *dw, const double *dh,

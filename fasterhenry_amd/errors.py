@@ -66,3 +66,8 @@ class PfftError(SolveError):
 class InvalidExcitation(SolveError):
     """The excitation does not have one finite entry per port
     (SolveError::InvalidExcitation): index0 = ports, index1 = entries given."""
+
+
+class DeckError(SolveError):
+    """A deck line the reader rejects (fasterhenry-cli/src/inp.rs:682-702
+    ParseError): index0 = the line (0 for the deck as a whole)."""

@@ -32,7 +32,8 @@ SYMBOLS = [
     "fh_pfft_get_stats", "fh_pfft_data", "fh_pfft_destroy", "fh_gmres_dense", "fh_solve_batch",
     "fh_discretize_tangent", "fh_grad", "fh_partial_inductance_derivative", "fh_currents",
     "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path", "fh_system_create_coupled",
-    "fh_partial_inductance_masked",
+    "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
+    "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
 ]
 
 _ERRORS = {
@@ -40,7 +41,7 @@ _ERRORS = {
     4: errors.KernelError, 5: errors.MeshError, 6: errors.InvalidGrading,
     7: errors.InvalidFrequency, 8: errors.Singular, 9: errors.NotConverged,
     10: errors.SolveError, 11: errors.SolveError, 12: errors.HipError, 13: errors.PfftError,
-    14: errors.InvalidExcitation,
+    14: errors.InvalidExcitation, 15: errors.DeckError,
 }
 
 
@@ -204,6 +205,22 @@ def lib() -> C.CDLL:
                                                C.POINTER(CouplingMask), C.POINTER(vp)]
         L.fh_partial_inductance_masked.restype = st
         L.fh_partial_inductance_masked.argtypes = [vp, sz, C.POINTER(PairMask), vp, vp]
+        L.fh_deck_parse.restype = st
+        L.fh_deck_parse.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
+        L.fh_deck_problem.restype = st
+        L.fh_deck_problem.argtypes = [vp, C.POINTER(Problem)]
+        L.fh_deck_info.restype = st
+        L.fh_deck_info.argtypes = [vp, C.POINTER(sz), C.POINTER(C.POINTER(C.c_double)),
+                                   C.POINTER(C.c_int32), C.POINTER(sz)]
+        L.fh_deck_coupling.restype = st
+        L.fh_deck_coupling.argtypes = [vp, C.POINTER(CouplingMask)]
+        for name in ("fh_deck_title", "fh_deck_port_name", "fh_deck_group_name"):
+            getattr(L, name).restype = C.c_char_p
+        L.fh_deck_title.argtypes = [vp]
+        L.fh_deck_port_name.argtypes = [vp, sz]
+        L.fh_deck_group_name.argtypes = [vp, sz]
+        L.fh_deck_destroy.restype = None
+        L.fh_deck_destroy.argtypes = [vp]
         L.fh_zc_mat.restype = st
         L.fh_zc_mat.argtypes = [vp, sz, vp, sz, vp, sz, C.POINTER(sz)]
         L.fh_spice_subckt.restype = st

@@ -48,8 +48,10 @@ typedef enum fh_status {
   FH_E_HIP = 12,             /* HIP runtime error */
   FH_E_PFFT = 13,            /* PfftError (pfft/mod.rs:236-251): code 1 = InvalidParameter,
                                 2 = GridTooLarge (index0 = padded points, index1 = limit) */
-  FH_E_INVALID_EXCITATION = 14 /* SolveError::InvalidExcitation: index0 = ports expected,
-                                  index1 = entries given */
+  FH_E_INVALID_EXCITATION = 14, /* SolveError::InvalidExcitation: index0 = ports expected,
+                                   index1 = entries given */
+  FH_E_DECK = 15               /* deck ParseError (fasterhenry-cli/src/inp.rs:682-702):
+                                  index0 = line (0: the deck as a whole) */
 } fh_status;
 
 /* SolverChoice (solve.rs:169-189) and Solver (solve.rs:143-158). */
@@ -389,6 +391,30 @@ fh_status fh_system_create_coupled(const fh_problem *problem, fh_solver_choice c
  * be NULL (every pair); methods (may be NULL) holds -1 for truncated pairs. */
 fh_status fh_partial_inductance_masked(const fh_filament *filaments, size_t n,
                                        const fh_pair_mask *mask, double *L_out, int8_t *methods);
+
+/* The FastHenry `.inp` deck (fasterhenry-cli/src/inp.rs, parse :3447-3625),
+ * conductor subset: .title, .units (mandatory), .default, N nodes, E
+ * segments (w h nwinc nhinc rw rh wx wy wz sigma|rho group), .external,
+ * .freq, .equiv, .couples, .end; ground planes (G, .hole, .contact) return
+ * FH_E_DECK.  fh_deck_problem points into the deck (valid until
+ * fh_deck_destroy): the nodes segments or ports reference, one grid per
+ * segment, the ports.  fh_deck_info: the sweep (Hz), the discretization
+ * (0 Uniform, 1 PerSegment, 2 PerSegmentGraded: discretization_of,
+ * inp.rs:4733-4765) and the number of coupling groups.  fh_deck_coupling:
+ * the resolved `.couples` truncation for fh_system_create_coupled
+ * (coupled = NULL: all pairs).  Names: port labels (an explicit label as
+ * written, else "<+>/<->" lowercased), group names, the title (NULL if
+ * none). */
+typedef struct fh_deck fh_deck;
+fh_status fh_deck_parse(const char *text, size_t length, fh_deck **out);
+fh_status fh_deck_problem(const fh_deck *deck, fh_problem *out);
+fh_status fh_deck_info(const fh_deck *deck, size_t *n_frequencies, const double **frequencies,
+                       int32_t *discretization, size_t *n_groups);
+fh_status fh_deck_coupling(const fh_deck *deck, fh_coupling *out);
+const char *fh_deck_title(const fh_deck *deck);
+const char *fh_deck_port_name(const fh_deck *deck, size_t port);
+const char *fh_deck_group_name(const fh_deck *deck, size_t group);
+void fh_deck_destroy(fh_deck *deck);
 
 /* Kernel level.  partial_inductance_matrix (inductance/batch.rs:246-253):
  * L_out (n x n, row-major, host) for host filaments.  methods (may be NULL)
