@@ -352,6 +352,11 @@ class MeshSystem:
     def set_profiling(self, enable: bool) -> None:
         native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
 
+    def set_sweep_parts(self, parts: int) -> None:
+        """Refined dense sweeps: factorize a chunk of frequencies as 1, 2 or
+        4 groups on as many streams (default 2)."""
+        native.check(native.lib().fh_system_set_sweep_parts(self._h, int(parts)))
+
     def counts(self) -> Counts:
         return self._counts
 

@@ -143,6 +143,14 @@ struct fh_system {
   DeviceArray<int32_t> d_ree_col;
   DeviceArray<double> d_ree_val, d_rep, d_rpp;
   LdltWorkspace ws;
+  // more streams and workspaces: a refined sweep chunk is solved as a few
+  // parts side by side, so that one part's latency-bound kernels (the
+  // diagonal blocks, panels, triangular solves) run under the others' bulk
+  // updates
+  static constexpr int kParts = 4;
+  int sweep_parts = 2;  // two measured fastest on MI355X (four contend)
+  hipStream_t part_stream[kParts] = {nullptr, nullptr, nullptr, nullptr};
+  LdltWorkspace part_ws[kParts];
   double inductance_s = 0.0, assembly_s = 0.0;
   // a copy of the problem (geometry tangents for gradients)
   std::vector<double> prob_nodes;
@@ -172,6 +180,10 @@ struct fh_system {
     d_L.release();
     d_lmesh.release();
     ws.release();
+    for (int q = 0; q < kParts; ++q) {
+      part_ws[q].release();
+      if (part_stream[q]) (void)hipStreamDestroy(part_stream[q]);
+    }
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -214,6 +226,9 @@ fh_status check_device() {
   }
   return FH_OK;
 }
+
+// Frequencies per part of a split refined sweep chunk, at least.
+constexpr int kSplitMin = 4;
 
 size_t sweep_budget_bytes() {
   size_t free_b = 0, total_b = 0;
@@ -698,6 +713,8 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
   }
   Profiler prof;
   Profiler *pp = s->profiling ? &prof : nullptr;
+  double extra_ms = 0.0, extra_flops = 0.0, extra_unit_flops = 0.0;  // the parts' profiled launches
+  size_t extra_launches = 0;
   int refine_steps_max = 0;
   double refine_change_max = 0.0;
   int factor_arith = s->precision == FH_FP32_REFINED ? 1 : 0;
@@ -745,10 +762,73 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       FH_HIP_TRY(hipMemcpyAsync(d_omegas.ptr, omegas.data(), kc * sizeof(double),
                                 hipMemcpyHostToDevice, s->stream));
       BatchResult res;
-      st = refined ? ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
-                                        s->stream, pp)
-                   : ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
-                                      s->stream, pp);
+      // parts side by side: a refined chunk of enough frequencies on a
+      // system large enough for the parts to fill the device, outside the
+      // batch workers (which already run one system per stream)
+      const int want = s->sweep_parts;
+      const int parts = !(refined && m >= 1024 && !tl_worker_stream && s->own_stream) ? 1
+                        : kc >= static_cast<size_t>(want * kSplitMin) ? want
+                        : kc >= static_cast<size_t>(2 * kSplitMin) ? std::min(2, want) : 1;
+      if (parts > 1) {
+        FH_HIP_TRY(hipStreamSynchronize(s->stream));  // the frequencies are uploaded
+        int dev = 0;
+        FH_HIP_TRY(hipGetDevice(&dev));
+        std::vector<size_t> off(parts + 1, 0);
+        for (int q = 0; q < parts; ++q) off[q + 1] = off[q] + (kc - off[q]) / (parts - q);
+        std::vector<BatchResult> rs(parts);
+        std::vector<Profiler> profs(parts);
+        std::vector<fh_status> sts(parts, FH_OK);
+        std::vector<ErrorState> errs(parts);
+        std::vector<double> ms(parts, 0.0);
+        std::vector<size_t> launches(parts, 0);
+        for (int q = 1; q < parts; ++q)
+          if (!s->part_stream[q]) FH_HIP_TRY(hipStreamCreateWithFlags(&s->part_stream[q], hipStreamNonBlocking));
+        auto run_part = [&](int q) {
+          hipStream_t sq = q == 0 ? s->stream : s->part_stream[q];
+          LdltWorkspace &wq = q == 0 ? s->ws : s->part_ws[q];
+          const size_t k0 = off[q], kn = off[q + 1] - off[q];
+          fh_status stq = q == 0 || hipSetDevice(dev) == hipSuccess
+                              ? ldlt_refined_batch(in, d_omegas.ptr + k0, static_cast<int>(kn), d_z.ptr + 2 * p * p * k0,
+                                                   rs[q], wq, sq, pp ? &profs[q] : nullptr)
+                              : set_error(FH_E_HIP, "hipSetDevice failed in a sweep part");
+          if (!stq && hipStreamSynchronize(sq) != hipSuccess)
+            stq = set_error(FH_E_HIP, "hipStreamSynchronize failed on a sweep part's stream");
+          if (!stq && pp) {
+            ms[q] = profs[q].total_ms();
+            launches[q] = profs[q].used;
+          }
+          if (stq) errs[q] = error_state();
+          sts[q] = stq;
+        };
+        std::vector<std::thread> workers;
+        for (int q = 1; q < parts; ++q) workers.emplace_back(run_part, q);
+        run_part(0);
+        for (auto &t : workers) t.join();
+        for (int q = 0; q < parts; ++q)
+          if (sts[q]) {
+            error_state() = errs[q];
+            return sts[q];
+          }
+        res = BatchResult{};
+        for (int q = 0; q < parts; ++q) {
+          res.singular.insert(res.singular.end(), rs[q].singular.begin(), rs[q].singular.end());
+          res.refine_steps.insert(res.refine_steps.end(), rs[q].refine_steps.begin(), rs[q].refine_steps.end());
+          res.refine_change.insert(res.refine_change.end(), rs[q].refine_change.begin(), rs[q].refine_change.end());
+          res.converged.insert(res.converged.end(), rs[q].converged.begin(), rs[q].converged.end());
+          res.factor_arith = std::max(res.factor_arith, rs[q].factor_arith);
+          if (pp) {
+            extra_ms += ms[q];
+            extra_launches += launches[q];
+            extra_flops += profs[q].flops;
+            extra_unit_flops += profs[q].unit_flops;
+          }
+        }
+      } else {
+        st = refined ? ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
+                                          s->stream, pp)
+                     : ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
+                                        s->stream, pp);
+      }
       if (st) return st;
       FH_HIP_TRY(hipMemcpyAsync(z_out + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s->stream));
@@ -788,8 +868,10 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->refine_steps_max = refine_steps_max;
     info->factor_arith = factor_arith;
     info->refine_change_max = refine_change_max;
-    info->update_kernel_ms = pp ? prof.total_ms() : 0.0;
-    info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used) : 0;
+    info->update_kernel_ms = pp ? prof.total_ms() + extra_ms : 0.0;
+    info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used + extra_launches) : 0;
+    prof.flops += extra_flops;
+    prof.unit_flops += extra_unit_flops;
     info->update_kernel_flops = pp ? prof.flops : 0.0;
     info->update_kernel_unit_flops = pp ? prof.unit_flops : 0.0;
     info->solver = FH_SOLVER_DENSE;
@@ -799,6 +881,14 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->operator_applies = 0;
     info->operator_apply_ms = 0.0;
   }
+  return FH_OK;
+}
+
+fh_status fh_system_set_sweep_parts(fh_system *s, int32_t parts) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  if (parts != 1 && parts != 2 && parts != 4) return set_error(FH_E_INVALID_ARGUMENT, "parts must be 1, 2 or 4");
+  s->sweep_parts = parts;
   return FH_OK;
 }
 

@@ -34,6 +34,7 @@ SYMBOLS = [
     "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path", "fh_system_create_coupled",
     "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
     "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
+    "fh_system_set_sweep_parts",
 ]
 
 _ERRORS = {
@@ -205,6 +206,8 @@ def lib() -> C.CDLL:
                                                C.POINTER(CouplingMask), C.POINTER(vp)]
         L.fh_partial_inductance_masked.restype = st
         L.fh_partial_inductance_masked.argtypes = [vp, sz, C.POINTER(PairMask), vp, vp]
+        L.fh_system_set_sweep_parts.restype = st
+        L.fh_system_set_sweep_parts.argtypes = [vp, C.c_int32]
         L.fh_deck_parse.restype = st
         L.fh_deck_parse.argtypes = [C.c_char_p, sz, C.POINTER(vp)]
         L.fh_deck_problem.restype = st

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fasterhenry_amd import (InvalidFrequency, MeshSystem, fixtures, model,
+from fasterhenry_amd import (InvalidArgument, InvalidFrequency, MeshSystem, fixtures, model,
                              partial_inductance_matrix, device_math)
 
 pytestmark = pytest.mark.gpu
@@ -178,4 +178,33 @@ def test_meander_2400_refined_parity():
     assert system.counts().internal_meshes > 512
     assert res.extra["factor_arith"] == 2
     assert res.extra["refine_steps_max"] <= 4
+    system.close()
+
+
+@pytest.mark.parametrize("parts", [1, 2, 4])
+def test_split_sweep_matches_one_frequency_at_a_time(parts):
+    # a refined chunk of 8 (16) or more frequencies on m >= 1024 is solved as
+    # two (four) groups on as many streams; each frequency's Z is that of its
+    # own sweep
+    g, ports, d = fixtures.meander_problem(24)
+    freqs = fixtures.log_frequencies(1e4, 1e9, 17)
+    system = MeshSystem.assemble(g, ports, d, precision="fp32_refined")
+    system.set_sweep_parts(parts)
+    assert system.counts().internal_meshes >= 1024
+    z = system.sweep(freqs).impedance_ohm
+    for k in (0, 5, 8, 12, 16):
+        z1 = system.sweep([freqs[k]]).impedance_ohm[0]
+        assert rel_frobenius(z[k], z1) <= 1e-12, (k, freqs[k])
+    system.set_precision("fp64")
+    z64 = system.sweep(freqs).impedance_ohm
+    assert max(rel_frobenius(z[k], z64[k]) for k in range(len(freqs))) <= Z_TOL_REFINED
+    system.close()
+
+
+def test_sweep_parts_argument():
+    g, ports, d = fixtures.meander_problem(2)
+    system = MeshSystem.assemble(g, ports, d, precision="fp32_refined")
+    for bad in (0, 3, 5, -1):
+        with pytest.raises(InvalidArgument):
+            system.set_sweep_parts(bad)
     system.close()
