@@ -301,6 +301,25 @@ def adjoint(steps: int, warmup: int, fd_check: bool = True, cpu: bool = False) -
     return out
 
 
+def plain_copy(mib: int, repeats: int = 5) -> dict:
+    """A device-to-device copy of known size: FETCH_SIZE and WRITE_SIZE of
+    its dispatches give the scale from the counters' units to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / repeats
+    assert torch.equal(x, y)
+    return {"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "repeats": repeats,
+            "ms_per_copy": dt * 1e3, "gb_per_s": 8 * n / dt / 1e9}
+
+
 def model_pack(problem):
     from fasterhenry_amd import model
 
@@ -329,11 +348,19 @@ def main() -> int:
                     help="leave the adjoint gradients out of the headline line")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint workload: skip the central-difference check")
+    ap.add_argument("--sweep-parts", type=int, default=2, choices=[1, 2, 4],
+                    help="refined sweep: groups of frequencies factorized side by side on as many streams")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter scale for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1",
                           "n_gpus": args.gpus}))
         return 2
+
+    if args.copy_mib:
+        print(json.dumps(plain_copy(args.copy_mib)))
+        return 0
 
     from fasterhenry_amd import MeshSystem, fixtures, model, native
 
@@ -357,6 +384,7 @@ def main() -> int:
     t0 = time.perf_counter()
     system = MeshSystem.assemble_packed(packed, ports, args.precision)
     assembly_s = time.perf_counter() - t0
+    system.set_sweep_parts(args.sweep_parts)
     counts = system.counts()
     print(f"# assembled in {assembly_s:.2f} s: m={counts.internal_meshes} p={counts.ports}",
           flush=True)
@@ -441,7 +469,13 @@ def main() -> int:
                      "kernel": "outer trailing update of the L D L^T factorization "
                                f"({peak_kind} matrix cores)",
                      "achieved_equivalent_tflops": achieved_equiv,
-                     "launches": upd_n, "avg_launch_ms": upd_ms / max(upd_n, 1)},
+                     "launches": upd_n, "avg_launch_ms": upd_ms / max(upd_n, 1),
+                     "streams": args.sweep_parts if args.precision != "fp64" else 1,
+                     # with the parts' launches overlapping, a launch's own
+                     # duration stretches; this is every update flop of the
+                     # timed steps over their whole wall time
+                     "device_achieved": upd_unit_flops / elapsed / 1e12,
+                     "device_frac": upd_unit_flops / elapsed / 1e12 / peak},
         "cpu_baseline": cpu,
         **extra,
     }

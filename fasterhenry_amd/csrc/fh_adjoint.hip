@@ -28,6 +28,7 @@
 
 #include "fh_adjoint.hpp"
 #include "fh_math.hpp"
+#include "fh_wave.hpp"
 
 namespace fh {
 namespace {
@@ -238,16 +239,12 @@ struct Aligned {
 // inductance/mod.rs:280-300 (the fill's aligned_of), on duals.
 __device__ bool aligned_of(const Bar &a, const Bar &b, Aligned &o) {
   const double kAligned = 1.0 - 1e-12;
-  Dual wb, hb;
-  if (__builtin_fabs(dot(a.axes[1], b.axes[1]).v) > kAligned) {
-    wb = b.half[1];
-    hb = b.half[2];
-  } else if (__builtin_fabs(dot(a.axes[1], b.axes[2]).v) > kAligned) {
-    wb = b.half[2];
-    hb = b.half[1];
-  } else {
-    return false;
-  }
+  // values, not a choice between addresses (which would put the bar in
+  // scratch memory)
+  const Dual h1 = b.half[1], h2 = b.half[2];
+  const bool same = __builtin_fabs(dot(a.axes[1], b.axes[1]).v) > kAligned;
+  if (!same && !(__builtin_fabs(dot(a.axes[1], b.axes[2]).v) > kAligned)) return false;
+  const Dual wb{same ? h1.v : h2.v, same ? h1.d : h2.d}, hb{same ? h2.v : h1.v, same ? h2.d : h1.d};
   const V3 delta = b.centre - a.centre;
   o.length[0] = 2.0 * a.half[0];
   o.length[1] = 2.0 * b.half[0];
@@ -444,13 +441,14 @@ __device__ Dual primitive(Dual x, Dual y, Dual z) {
          atan_term(x, z, y, r) + atan_term(y, z, x, r);
 }
 
-__device__ __forceinline__ void corner_differences(Dual e1, Dual e2, Dual off, Dual q[4]) {
+// Corner coordinate c (the fill's corner, on duals), chosen by value.
+__device__ __forceinline__ Dual corner(Dual e1, Dual e2, Dual off, int c) {
   const Dual sum = 0.5 * (e1 + e2), diff = 0.5 * (e1 - e2);
-  q[0] = off + sum;
-  q[1] = off + diff;
-  q[2] = off - diff;
-  q[3] = off - sum;
+  const bool outer = c == 0 || c == 3;
+  const Dual t{outer ? sum.v : diff.v, outer ? sum.d : diff.d};
+  return c < 2 ? off + t : off - t;
 }
+__device__ __forceinline__ double corner_sign(int c) { return (c == 0 || c == 3) ? 1.0 : -1.0; }
 
 // ---------------------------------------------------------------------------
 // lines.rs
@@ -749,14 +747,12 @@ __device__ Dual aligned_quadrature_wave(const Aligned &b, Frame *stack, Dual *ld
 }
 
 __device__ Dual aligned_dispatch_wave(const Aligned &b, Frame *stack, Dual *lds) {
-  Dual qx[4], qy[4], qz[4];
-  corner_differences(b.length[0], b.length[1], b.offset[0], qx);
-  corner_differences(b.width[0], b.width[1], b.offset[1], qy);
-  corner_differences(b.height[0], b.height[1], b.offset[2], qz);
   const int lane = static_cast<int>(threadIdx.x);
-  const double signs[4] = {1.0, -1.0, -1.0, 1.0};
   const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
-  const Dual term = signs[i] * signs[j] * signs[k] * primitive(qx[i], qy[j], qz[k]);
+  const Dual qx = corner(b.length[0], b.length[1], b.offset[0], i);
+  const Dual qy = corner(b.width[0], b.width[1], b.offset[1], j);
+  const Dual qz = corner(b.height[0], b.height[1], b.offset[2], k);
+  const Dual term = corner_sign(i) * corner_sign(j) * corner_sign(k) * primitive(qx, qy, qz);
   __syncthreads();
   lds[lane] = term;
   __syncthreads();
@@ -772,7 +768,7 @@ __device__ Dual aligned_dispatch_wave(const Aligned &b, Frame *stack, Dual *lds)
 }
 
 // The fill's deferred evaluation (one wave), on duals: 1e-7 x integral.
-__device__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual *lds, Frame *stack, int lane, bool &ok) {
+__device__ __forceinline__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual *lds, Frame *stack, int lane, bool &ok) {
   Dual integral = cst(0.0);
   ok = true;
   if (kind == kHeavyPoint) {
@@ -930,6 +926,8 @@ __global__ void __launch_bounds__(256) dtile_kernel(const fh_filament *__restric
   const int f = ti * kTile + static_cast<int>(threadIdx.x) / kTile;
   const int g = tj * kTile + static_cast<int>(threadIdx.x) % kTile;
   double d = 0.0;
+  bool queued = false;
+  uint32_t kind = 0;
   if (f < n && g < n && f <= g) {
     const size_t qn = static_cast<size_t>(q) * n;
     const bool moves = moving[qn + f] || moving[qn + g];
@@ -941,12 +939,15 @@ __global__ void __launch_bounds__(256) dtile_kernel(const fh_filament *__restric
         if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
           if (moves) d = (1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos)).d;
         } else if (q == 0) {
-          const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled);
-          const unsigned slot = atomicAdd(qcount, 1u);
-          queue[slot] = make_uint2(static_cast<uint32_t>(f) | (kind << 30), static_cast<uint32_t>(g));
+          kind = p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled);
+          queued = true;
         }
       }
     }
+  }
+  {
+    const unsigned slot = wave_append(queued, qcount);
+    if (queued) queue[slot] = make_uint2(static_cast<uint32_t>(f) | (kind << 30), static_cast<uint32_t>(g));
   }
   auto term = [&](int o) {
     if (d == 0.0) return make_double2(0.0, 0.0);

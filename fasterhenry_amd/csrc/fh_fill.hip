@@ -33,6 +33,7 @@
 
 #include "fh_internal.hpp"
 #include "fh_math.hpp"
+#include "fh_wave.hpp"
 
 namespace fh {
 namespace {
@@ -249,16 +250,12 @@ struct Aligned {
 // inductance/mod.rs:280-300
 __device__ bool aligned_of(const Bar &a, const Bar &b, Aligned &o) {
   const double kAligned = 1.0 - 1e-12;
-  double wb, hb;
-  if (__builtin_fabs(dot(a.axes[1], b.axes[1])) > kAligned) {
-    wb = b.half[1];
-    hb = b.half[2];
-  } else if (__builtin_fabs(dot(a.axes[1], b.axes[2])) > kAligned) {
-    wb = b.half[2];
-    hb = b.half[1];
-  } else {
-    return false;
-  }
+  // values, not a choice between addresses: an address picked at run time
+  // would put the whole bar in scratch memory
+  const double h1 = b.half[1], h2 = b.half[2];
+  const bool same = __builtin_fabs(dot(a.axes[1], b.axes[1])) > kAligned;
+  if (!same && !(__builtin_fabs(dot(a.axes[1], b.axes[2])) > kAligned)) return false;
+  const double wb = same ? h1 : h2, hb = same ? h2 : h1;
   const D3 delta = b.centre - a.centre;
   o.length[0] = 2.0 * a.half[0];
   o.length[1] = 2.0 * b.half[0];
@@ -408,13 +405,16 @@ __device__ double primitive(double x, double y, double z) {
          atan_term(x, y, z, r) + atan_term(x, z, y, r) + atan_term(y, z, x, r);
 }
 
-__device__ __forceinline__ void corner_differences(double e1, double e2, double off, double q[4]) {
+// Corner coordinate c of one axis: off + sum, off + diff, off - diff,
+// off - sum for c = 0..3 (sum, diff = half the sum, difference of the two
+// extents), with sign +, -, -, +.  Chosen by value: a lane-dependent index
+// into an array would put the array in scratch memory.
+__device__ __forceinline__ double corner(double e1, double e2, double off, int c) {
   const double sum = 0.5 * (e1 + e2), diff = 0.5 * (e1 - e2);
-  q[0] = off + sum;
-  q[1] = off + diff;
-  q[2] = off - diff;
-  q[3] = off - sum;
+  const double t = (c == 0 || c == 3) ? sum : diff;
+  return c < 2 ? off + t : off - t;
 }
+__device__ __forceinline__ double corner_sign(int c) { return (c == 0 || c == 3) ? 1.0 : -1.0; }
 
 // ---------------------------------------------------------------------------
 // lines.rs
@@ -711,14 +711,12 @@ __device__ double aligned_quadrature_wave(const Aligned &b, Frame *stack, double
 // inductance/mod.rs:265-276 with bar.rs:84-106 on the lanes.
 __device__ double aligned_dispatch_wave(const Aligned &b, int8_t &method, Frame *stack,
                                         double *lds) {
-  double qx[4], qy[4], qz[4];
-  corner_differences(b.length[0], b.length[1], b.offset[0], qx);
-  corner_differences(b.width[0], b.width[1], b.offset[1], qy);
-  corner_differences(b.height[0], b.height[1], b.offset[2], qz);
   const int lane = static_cast<int>(threadIdx.x);
-  const double signs[4] = {1.0, -1.0, -1.0, 1.0};
   const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
-  const double term = signs[i] * signs[j] * signs[k] * primitive(qx[i], qy[j], qz[k]);
+  const double qx = corner(b.length[0], b.length[1], b.offset[0], i);
+  const double qy = corner(b.width[0], b.width[1], b.offset[1], j);
+  const double qz = corner(b.height[0], b.height[1], b.offset[2], k);
+  const double term = corner_sign(i) * corner_sign(j) * corner_sign(k) * primitive(qx, qy, qz);
   __syncthreads();
   lds[lane] = term;
   __syncthreads();
@@ -809,6 +807,8 @@ __global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__res
   double value = 0.0;
   int8_t method = kOrthogonal;
   bool finished = false;
+  bool queued = false;
+  uint32_t kind = 0;
   if (i < n && j < n && i <= j && !mask_keeps(mask, i, j)) {
     finished = true;  // truncated: exactly 0, never evaluated
     method = kTruncated;
@@ -824,10 +824,13 @@ __global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__res
       finished = true;
       if (!__builtin_isfinite(value)) record_error(err, i, j, n, 2);
     } else {
-      const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
-      const unsigned int slot = atomicAdd(queue_count, 1u);
-      queue[slot] = make_uint2(static_cast<uint32_t>(i) | (kind << 30), static_cast<uint32_t>(j));
+      kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
+      queued = true;
     }
+  }
+  {
+    const unsigned slot = wave_append(queued, queue_count);
+    if (queued) queue[slot] = make_uint2(static_cast<uint32_t>(i) | (kind << 30), static_cast<uint32_t>(j));
   }
   vals[r][c] = value;
   meth[r][c] = method;
@@ -977,21 +980,25 @@ __global__ void __launch_bounds__(256) pairs_kernel(const fh_filament *__restric
                                                     unsigned int *__restrict__ queue_count,
                                                     unsigned long long *__restrict__ err) {
   const size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (k >= count) return;
-  const int i = static_cast<int>(pairs[k].x), j = static_cast<int>(pairs[k].y);
-  PairPlan p;
-  plan_pair(F[i], F[j], p);
-  if (p.orthogonal) {
-    out[k] = 0.0;
-  } else if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
-    const double value = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
-    out[k] = value;
-    if (!__builtin_isfinite(value)) record_error(err, i, j, n, 2);
-  } else {
-    const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
-    const unsigned int slot = atomicAdd(queue_count, 1u);
-    queue[slot] = make_uint2(static_cast<uint32_t>(k) | (kind << 30), 0u);
+  bool queued = false;
+  uint32_t kind = 0;
+  if (k < count) {
+    const int i = static_cast<int>(pairs[k].x), j = static_cast<int>(pairs[k].y);
+    PairPlan p;
+    plan_pair(F[i], F[j], p);
+    if (p.orthogonal) {
+      out[k] = 0.0;
+    } else if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
+      const double value = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
+      out[k] = value;
+      if (!__builtin_isfinite(value)) record_error(err, i, j, n, 2);
+    } else {
+      kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
+      queued = true;
+    }
   }
+  const unsigned slot = wave_append(queued, queue_count);
+  if (queued) queue[slot] = make_uint2(static_cast<uint32_t>(k) | (kind << 30), 0u);
 }
 
 __global__ void __launch_bounds__(64) pairs_deferred_kernel(
