@@ -164,14 +164,44 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // costs each lane 16 complex multiply-adds and one barrier.  Pivot columns
 // reach the other waves through a two-entry LDS ring.  Linv is built in the
 // same pass: the row operations of the elimination (row r -= l_rc row c) are
-// applied to an identity X, which becomes L11^-1; X[c][j] is read from lane
-// c of the owning wave.
+// applied to an identity X, which becomes L11^-1.  Slot j holds A[r][j]
+// until column j is eliminated and X[r][j] from then on (X[r][j] is the
+// identity until that step), so each slot has one update: for j > c,
+// A[r][j] -= l_r A[j][c] with A[j][c] from the ring; for j < c,
+// X[r][j] -= l_r X[c][j] with X[c][j] read from lane c; for j = c the slot
+// becomes X[r][c] = delta_rc - l_r.
+//
+// The working type C is the factor's type: FP64 for the FP64 factorization,
+// FP32 for the FP32 one (whose products elsewhere carry split-bf16 rounding
+// far above FP32's, and which refinement in FP64 corrects).
 
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
+template <typename C>
+struct Zc {
+  C re, im;
+};
+template <typename C>
+__device__ __forceinline__ C fmac(C a, C b, C c) {
+  if constexpr (std::is_same<C, float>::value) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+template <typename C>
+__device__ __forceinline__ Zc<C> zmulc(Zc<C> a, Zc<C> b) {
+  return {fmac(a.re, b.re, -(a.im * b.im)), fmac(a.re, b.im, a.im * b.re)};
+}
+template <typename C>
+__device__ __forceinline__ Zc<C> zrecipc(Zc<C> d) {
+  const C n = d.re * d.re + d.im * d.im;
+  return {d.re / n, -d.im / n};
+}
+
+__device__ __forceinline__ double readlane_c(double v, int lane) {
   const uint64_t u = __builtin_bit_cast(uint64_t, v);
   const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffu), lane);
   const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane);
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float readlane_c(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
 template <typename T>
@@ -181,40 +211,39 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
                                                    T *__restrict__ linvt_im, T *__restrict__ dinv_re,
                                                    T *__restrict__ dinv_im,
                                                    int *__restrict__ singular) {
+  using C = T;
   const int f = blockIdx.x;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = threadIdx.x & 63;
   const int rc = min(r, w - 1);
-  __shared__ double cbr[2][kB], cbi[2][kB];
+  __shared__ C cbr[2][kB], cbi[2][kB];
   // L11 by columns, D and D^-1: written to memory after the loop, so that
   // no global store is outstanding at the barriers of the column steps
-  __shared__ double lsr[kB][kB], lsi[kB][kB], dsr[kB], dsi[kB], vsr[kB], vsi[kB];
-  double ar[16], ai[16], xr[16], xi[16];
+  __shared__ C lsr[kB][kB], lsi[kB][kB], dsr[kB], dsi[kB], vsr[kB], vsi[kB];
+  C mr[16], mi[16];
 #pragma unroll
   for (int js = 0; js < 16; ++js) {
     const int j = 4 * js + g;
     const bool in = r < w && j <= r;
     const int64_t a = static_cast<int64_t>(k0 + min(j, rc)) * ld + k0 + rc;
-    const double vr = static_cast<double>(Re[a]), vi = static_cast<double>(Im[a]);
-    ar[js] = in ? vr : 0.0;
-    ai[js] = in ? vi : 0.0;
-    xr[js] = (j == r) ? 1.0 : 0.0;
-    xi[js] = 0.0;
+    const C vr = Re[a], vi = Im[a];
+    mr[js] = in ? vr : C(0);
+    mi[js] = in ? vi : C(0);
   }
   if (g == 0) {
-    cbr[0][r] = ar[0];
-    cbi[0][r] = ai[0];
+    cbr[0][r] = mr[0];
+    cbi[0][r] = mi[0];
   }
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
   __syncthreads();
   for (int c = 0; c < w; ++c) {
     const int b = c & 1;
-    const Z2 d{cbr[b][c], cbi[b][c]};
-    const Z2 inv = zrecip(d);
+    const Zc<C> d{cbr[b][c], cbi[b][c]};
+    const Zc<C> inv = zrecipc(d);
     const bool below = r > c && r < w;
-    const Z2 lv = zmul(Z2{cbr[b][r], cbi[b][r]}, inv);
-    const Z2 l{below ? lv.re : 0.0, below ? lv.im : 0.0};
+    const Zc<C> lv = zmulc(Zc<C>{cbr[b][r], cbi[b][r]}, inv);
+    const Zc<C> l{below ? lv.re : C(0), below ? lv.im : C(0)};
     if (g == 0) {
       lsr[c][r] = l.re;
       lsi[c][r] = l.im;
@@ -225,31 +254,28 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
         vsi[c] = inv.im;
       }
     }
-    // Slot js, column j: A[r][j] -= l_r A[j][c] for j > c, else
-    // X[r][j] -= l_r X[c][j].  Written without branches (one product, the
-    // other array gets -0): a branch here makes the compiler copy the
-    // register arrays through every arm.
-    // all reads of the ring entry before its other entry is written
-    double bcr[16], bci[16];
+    // Written without branches on the slot's state (selects): a branch here
+    // makes the compiler copy the register arrays through every arm.
+    // All reads of the ring entry before its other entry is written.
+    C bcr[16], bci[16];
 #pragma unroll
     for (int js = 0; js < 16; ++js) {
       bcr[js] = cbr[b][4 * js + g];
       bci[js] = cbi[b][4 * js + g];
     }
+    const C delta = r == c ? C(1) : C(0);
 #pragma unroll
     for (int js = 0; js < 16; ++js) {
       const int j = 4 * js + g;
-      const bool upd_a = j > c;
-      const double br = bcr[js], bi = bci[js];
-      const double xcr = readlane_f64(xr[js], c), xci = readlane_f64(xi[js], c);
-      const Z2 pr = zmul(l, Z2{upd_a ? br : xcr, upd_a ? bi : xci});
-      ar[js] -= upd_a ? pr.re : 0.0;
-      ai[js] -= upd_a ? pr.im : 0.0;
-      xr[js] -= upd_a ? 0.0 : pr.re;
-      xi[js] -= upd_a ? 0.0 : pr.im;
+      const bool from_ring = j > c;
+      const C xcr = readlane_c(mr[js], c), xci = readlane_c(mi[js], c);
+      const Zc<C> pr = zmulc(l, Zc<C>{from_ring ? bcr[js] : xcr, from_ring ? bci[js] : xci});
+      const C ur = mr[js] - pr.re, ui = mi[js] - pr.im;
+      mr[js] = j == c ? delta - l.re : ur;
+      mi[js] = j == c ? C(0) - l.im : ui;
       if (j == c + 1) {  // column c + 1 goes to the other ring entry
-        cbr[b ^ 1][r] = ar[js];
-        cbi[b ^ 1][r] = ai[js];
+        cbr[b ^ 1][r] = mr[js];
+        cbi[b ^ 1][r] = mi[js];
       }
     }
     __syncthreads();
@@ -259,22 +285,22 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
   for (int js = 0; js < 16; ++js) {
     const int j = 4 * js + g;
     const bool in = r < w && j < w;
-    linvt_re[lbase + j * kB + r] = in ? static_cast<T>(xr[js]) : T(0);
-    linvt_im[lbase + j * kB + r] = in ? static_cast<T>(xi[js]) : T(0);
+    linvt_re[lbase + j * kB + r] = in ? mr[js] : T(0);
+    linvt_im[lbase + j * kB + r] = in ? mi[js] : T(0);
   }
   // L11 and D in place (column c, rows c..w-1), D^-1, the zero-pivot flag
   for (int q = threadIdx.x; q < kB * kB; q += blockDim.x) {
     const int c = q >> 6, rr = q & 63;
     if (c < w && rr >= c && rr < w) {
       const int64_t a = static_cast<int64_t>(k0 + c) * ld + k0 + rr;
-      Re[a] = static_cast<T>(rr == c ? dsr[c] : lsr[c][rr]);
-      Im[a] = static_cast<T>(rr == c ? dsi[c] : lsi[c][rr]);
+      Re[a] = rr == c ? dsr[c] : lsr[c][rr];
+      Im[a] = rr == c ? dsi[c] : lsi[c][rr];
     }
   }
   if (g == 0) {
-    dinv_re[dbase + r] = r < w ? static_cast<T>(vsr[r]) : T(0);
-    dinv_im[dbase + r] = r < w ? static_cast<T>(vsi[r]) : T(0);
-    if (r < w && dsr[r] == 0.0 && dsi[r] == 0.0) singular[f] = 1;
+    dinv_re[dbase + r] = r < w ? vsr[r] : T(0);
+    dinv_im[dbase + r] = r < w ? vsi[r] : T(0);
+    if (r < w && dsr[r] == C(0) && dsi[r] == C(0)) singular[f] = 1;
   }
 }
 
@@ -351,28 +377,57 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
   const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
   const int r0 = k0 + w + blockIdx.x * kT;
-  __shared__ T ps_r[kKC][kT], ps_i[kKC][kT], qs_r[kKC][kT], qs_i[kKC][kT];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wi = 32 * (wave >> 1), wj = 32 * (wave & 1);
+  const int kq = lane >> 4, x16 = lane & 15;
+  // Operands straight from memory into the matrix-core layout (lane l:
+  // P[i = l % 16][k = l / 16], Q[k = l / 16][j = l % 16]), a 16-deep chunk
+  // of k ahead of the products: no LDS staging and no barriers.  P is the
+  // inverse (zero outside the block), Q the panel rows; the products run
+  // in the order of the staged form, so W is unchanged.
+  struct Chunk {
+    T ar[4][2], ai[4][2], br[4][2], bi[4][2];
+  };
+  auto load = [&](int kc, Chunk &o) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = kc + 4 * q + kq;  // < kB: kc <= 48
+      const int64_t kcl = min(k, w - 1);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        o.ar[q][b] = linvt_re[lbase + k * kB + wi + 16 * b + x16];
+        o.ai[q][b] = linvt_im[lbase + k * kB + wi + 16 * b + x16];
+        const int rr = r0 + wj + 16 * b + x16;
+        const int64_t rcl = min(rr, N - 1);
+        const T a = Re[(k0 + kcl) * ld + rcl], c = Im[(k0 + kcl) * ld + rcl];
+        const bool ok = k < w && rr < N;
+        o.br[q][b] = ok ? a : T(0);
+        o.bi[q][b] = ok ? c : T(0);
+      }
+    }
+  };
   TileAcc<T> acc;
   tile_zero(acc);
-  for (int kc = 0; kc < w; kc += kKC) {
-    __syncthreads();
-    for (int q = threadIdx.x; q < kKC * kT; q += blockDim.x) {
-      const int k = q / kT, x = q % kT;
-      const int kk = kc + k;
-      ps_r[k][x] = linvt_re[lbase + kk * kB + x];  // zero outside the block
-      ps_i[k][x] = linvt_im[lbase + kk * kB + x];
-      const bool ok = kk < w && r0 + x < N;
-      const int64_t r = min(r0 + x, N - 1), kcl = min(kk, w - 1);
-      const T a = Re[(k0 + kcl) * ld + r], b = Im[(k0 + kcl) * ld + r];
-      qs_r[k][x] = ok ? a : T(0);
-      qs_i[k][x] = ok ? b : T(0);
-    }
-    __syncthreads();
-    tile_mma(acc, ps_r, ps_i, qs_r, qs_i, wi, wj, lane);
+  Chunk cur, nxt;
+  load(0, cur);
+#pragma unroll
+  for (int c = 0; c < kB / 16; ++c) {
+    const int kc = 16 * c;
+    if (kc >= w) break;
+    if (kc + 16 < w) load(kc + 16, nxt);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+          acc.re[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.br[q][jb], acc.re[ib][jb]);
+          acc.re[ib][jb] = Mfma<T>::run(-cur.ai[q][ib], cur.bi[q][jb], acc.re[ib][jb]);
+          acc.im[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.bi[q][jb], acc.im[ib][jb]);
+          acc.im[ib][jb] = Mfma<T>::run(cur.ai[q][ib], cur.br[q][jb], acc.im[ib][jb]);
+        }
+    cur = nxt;
   }
-  __syncthreads();
 #pragma unroll
   for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
