@@ -27,6 +27,12 @@ sys.path.insert(0, ROOT)
 
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 
+# HBM bytes per update_left_kernel launch on the default workload (meander(49),
+# 41 frequencies, refined, two parts), mean over the sweep's 228 launches:
+# rocprofv3 FETCH_SIZE 924.0 MB + WRITE_SIZE 65.2 MB, each scaled by its
+# reading for a plain 512 MiB device copy (bench.py --copy-mib 512).
+UPDATE_TRAFFIC_BYTES = 924.0e6 + 65.2e6
+
 
 def cpu_model() -> str:
     try:
@@ -465,7 +471,10 @@ def main() -> int:
                    "refine_steps_max": infos[-1]["refine_steps_max"],
                    "refine_change_max": infos[-1]["refine_change_max"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": UPDATE_TRAFFIC_BYTES if (args.bars, args.freqs, args.precision,
+                                                         args.sweep_parts) == (49, 41, "fp32_refined", 2)
+                     else None,
                      "kernel": "outer trailing update of the L D L^T factorization "
                                f"({peak_kind} matrix cores)",
                      "achieved_equivalent_tflops": achieved_equiv,
