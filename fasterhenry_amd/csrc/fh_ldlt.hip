@@ -640,7 +640,8 @@ __device__ __forceinline__ bool left_tile(int b, int rtiles, int ctiles, int &tr
 __global__ void __launch_bounds__(256) update_left_kernel(
     float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int J,
     int nbj, int N, int G, int rtiles, int ctiles, int kc_beg, int kc_end,
-    const bf16x8 *__restrict__ lsplit, const bf16x8 *__restrict__ wsplit, int64_t split_stride) {
+    const bf16x8 *__restrict__ lsplit, const bf16x8 *__restrict__ wsplit, int64_t split_stride,
+    int narrow) {
   const int f = blockIdx.y;
   int tr, tc;
   if (!left_tile(blockIdx.x, rtiles, ctiles, tr, tc)) return;
@@ -649,8 +650,10 @@ __global__ void __launch_bounds__(256) update_left_kernel(
   // registers to the operands and accumulators.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int wi = 64 * (wave >> 1), wj = 64 * (wave & 1);
-  const int s0 = J + 128 * tc + wi, r0 = J + 128 * tr + wj;
+  // 128 x 128 tiles (2 x 2 waves), or for a block of at most 64 columns
+  // 256 x 64 tiles (the four waves down the rows, sharing the L operand)
+  const int wi = narrow ? 0 : 64 * (wave >> 1), wj = narrow ? 64 * wave : 64 * (wave & 1);
+  const int s0 = J + 128 * tc + wi, r0 = J + (narrow ? 256 : 128) * tr + wj;
   // Operands through buffer loads: one descriptor per operand and
   // frequency (scalar), a fixed per-lane offset, the block offset in a
   // scalar register -- no per-lane 64-bit address arithmetic.
@@ -1286,12 +1289,15 @@ fh_status launch_left_update(const Layout &L, const Factor<float> &F, const Spli
   // the kernel takes k in pairs of 32-column chunks
   if (kc_end <= kc_beg || (kc_end - kc_beg) % 2 != 0)
     return set_error(FH_E_INVALID_ARGUMENT, "left-looking update: odd chunk range");
-  const int rtiles = (L.N - J + 127) / 128, ctiles = (nbj + 127) / 128;
+  // a block of at most 64 columns: 256-row tiles one block wide
+  const int narrow = nbj <= 64 ? 1 : 0;
+  const int rtiles = narrow ? (L.N - J + 255) / 256 : (L.N - J + 127) / 128;
+  const int ctiles = narrow ? 1 : (nbj + 127) / 128;
   const int T = ctiles * (ctiles - 1) / 2 + (rtiles - (ctiles - 1)) * ctiles;
   if (prof) prof->begin(stream);
   update_left_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
       F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, kc_beg, kc_end, sb.l,
-      sb.w, sb.stride);
+      sb.w, sb.stride, narrow);
   if (prof) {
     double entries = 0.0;
     for (int s = J; s < J + nbj; ++s) entries += static_cast<double>(L.N - s);
