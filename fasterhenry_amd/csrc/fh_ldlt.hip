@@ -406,6 +406,16 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
       }
     }
   };
+  // D^-1 of this lane's output columns, loaded ahead of the products
+  T dvr[2][4], dvi[2][4];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = wi + 16 * ib + Mfma<T>::row(lane, v);
+      dvr[ib][v] = dinv_re[dbase + c];
+      dvi[ib][v] = dinv_im[dbase + c];
+    }
   TileAcc<T> acc;
   tile_zero(acc);
   Chunk cur, nxt;
@@ -442,7 +452,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
             Wr[(wcol + c) * ld + r] = wr;
             Wi[(wcol + c) * ld + r] = wim_;
           }
-          const T dr = dinv_re[dbase + c], di = dinv_im[dbase + c];
+          const T dr = dvr[ib][v], di = dvi[ib][v];
           Re[(k0 + c) * ld + r] = wr * dr - wim_ * di;
           Im[(k0 + c) * ld + r] = wr * di + wim_ * dr;
         }
@@ -465,7 +475,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
               const float wr = acc.re[ib][jb][v], wim_ = acc.im[ib][jb][v];
               float vr = wr, vi = wim_;
               if (part == 1) {
-                const float dr = dinv_re[dbase + wi + cc], di = dinv_im[dbase + wi + cc];
+                const float dr = dvr[ib][v], di = dvi[ib][v];
                 vr = wr * dr - wim_ * di;
                 vi = wr * di + wim_ * dr;
               }
