@@ -126,14 +126,20 @@ __host__ __device__ __forceinline__ int64_t split_block(int kc, int g, int G) {
 // frequencies of the batch per thread: l_mesh is read once.  l_mesh is
 // symmetric, so its row c is read for column c of A (row-contiguous).  Only
 // the lower triangle (r >= c, r < N) is written; no kernel reads the rest.
+// Row y of the grid covers column y (N - y entries) followed by column
+// N - 1 - y (y + 1 entries): N + 1 entries for every y, no idle workgroups.
 
 template <typename T>
 __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
                             const double *__restrict__ omegas, int K, int N, int64_t ld,
                             T *__restrict__ re, T *__restrict__ im, int64_t mat_stride) {
-  const int c = blockIdx.y;
-  const int r = c + blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= N) return;
+  const int y = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool first = i < N - y;
+  const int c = first ? y : N - 1 - y;
+  const int r = first ? y + i : c + (i - (N - y));
+  // the last row of an odd N pairs its column with itself: once
+  if (i > N || r >= N || (!first && c == y)) return;
   const double l = lmesh[c * loops + r];
   const int64_t a = c * ld + r;
   for (int f = 0; f < K; ++f) {
@@ -1326,7 +1332,7 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
                     const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
                     const SplitBufs *sb = nullptr) {
   {
-    dim3 grid((L.N + 255) / 256, L.N);
+    dim3 grid((L.N + 1 + 255) / 256, (L.N + 1) / 2);
     form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, K,
                                              L.N, L.ld, F.re, F.im, L.mat_stride);
     FH_HIP_TRY(hipGetLastError());
