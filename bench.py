@@ -354,7 +354,7 @@ def main() -> int:
                     help="leave the adjoint gradients out of the headline line")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint workload: skip the central-difference check")
-    ap.add_argument("--sweep-parts", type=int, default=2, choices=[1, 2, 4],
+    ap.add_argument("--sweep-parts", type=int, default=2, choices=[1, 2, 3, 4],
                     help="refined sweep: groups of frequencies factorized side by side on as many streams")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale for rocprofv3)")

@@ -353,8 +353,9 @@ class MeshSystem:
         native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
 
     def set_sweep_parts(self, parts: int) -> None:
-        """Refined dense sweeps: factorize a chunk of frequencies as 1, 2 or
-        4 groups on as many streams (default 2)."""
+        """Refined dense sweeps: factorize a chunk of frequencies as 1 to 4
+        groups on as many streams (default 2; a group has at least 4
+        frequencies)."""
         native.check(native.lib().fh_system_set_sweep_parts(self._h, int(parts)))
 
     def counts(self) -> Counts:

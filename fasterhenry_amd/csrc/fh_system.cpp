@@ -120,6 +120,9 @@ void block_cache(bool on) {
 
 using namespace fh;
 
+// Groups of frequencies a refined sweep may factorize side by side.
+constexpr int kSweepPartsMax = 4;
+
 struct fh_system {
   hipStream_t stream = nullptr;
   bool own_stream = true;
@@ -147,7 +150,7 @@ struct fh_system {
   // parts side by side, so that one part's latency-bound kernels (the
   // diagonal blocks, panels, triangular solves) run under the others' bulk
   // updates
-  static constexpr int kParts = 4;
+  static constexpr int kParts = kSweepPartsMax;
   int sweep_parts = 2;  // two measured fastest on MI355X (four contend)
   hipStream_t part_stream[kParts] = {nullptr, nullptr, nullptr, nullptr};
   LdltWorkspace part_ws[kParts];
@@ -767,8 +770,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       // batch workers (which already run one system per stream)
       const int want = s->sweep_parts;
       const int parts = !(refined && m >= 1024 && !tl_worker_stream && s->own_stream) ? 1
-                        : kc >= static_cast<size_t>(want * kSplitMin) ? want
-                        : kc >= static_cast<size_t>(2 * kSplitMin) ? std::min(2, want) : 1;
+                        : std::max(1, std::min<int>(want, static_cast<int>(kc / kSplitMin)));
       if (parts > 1) {
         FH_HIP_TRY(hipStreamSynchronize(s->stream));  // the frequencies are uploaded
         int dev = 0;
@@ -887,7 +889,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
 fh_status fh_system_set_sweep_parts(fh_system *s, int32_t parts) {
   clear_error();
   if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
-  if (parts != 1 && parts != 2 && parts != 4) return set_error(FH_E_INVALID_ARGUMENT, "parts must be 1, 2 or 4");
+  if (parts < 1 || parts > kSweepPartsMax) return set_error(FH_E_INVALID_ARGUMENT, "parts must be 1 to 4");
   s->sweep_parts = parts;
   return FH_OK;
 }

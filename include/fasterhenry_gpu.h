@@ -240,7 +240,7 @@ fh_status fh_system_counts(const fh_system *system, size_t counts[6]);
  * l_ep m x p, l_pp p x p, r_ee m x m, r_ep m x p, r_pp p x p. */
 fh_status fh_system_filaments(const fh_system *system, fh_filament *out);
 /* Scheduling of refined dense sweeps (no counterpart in the reference):
- * a chunk of frequencies is factorized as `parts` (1, 2 or 4) groups on as
+ * a chunk of frequencies is factorized as `parts` (1 to 4) groups on as
  * many streams, side by side, so that the latency-bound kernels of one
  * group overlap the bulk updates of the others.  Default 2.  Results do not
  * depend on it beyond the refinement's rounding (each frequency is refined
