@@ -29,9 +29,9 @@ PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 
 # HBM bytes per update_left_kernel launch on the default workload (meander(49),
 # 41 frequencies, refined, two parts), mean over the sweep's 228 launches:
-# rocprofv3 FETCH_SIZE 924.0 MB + WRITE_SIZE 65.2 MB, each scaled by its
+# rocprofv3 FETCH_SIZE 885.6 MB + WRITE_SIZE 65.2 MB, each scaled by its
 # reading for a plain 512 MiB device copy (bench.py --copy-mib 512).
-UPDATE_TRAFFIC_BYTES = 924.0e6 + 65.2e6
+UPDATE_TRAFFIC_BYTES = 885.6e6 + 65.2e6
 
 
 def cpu_model() -> str:
