@@ -803,8 +803,16 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
           sts[q] = stq;
         };
         std::vector<std::thread> workers;
-        for (int q = 1; q < parts; ++q) workers.emplace_back(run_part, q);
+        std::vector<int> here;  // parts whose thread could not be started
+        for (int q = 1; q < parts; ++q) {
+          try {
+            workers.emplace_back(run_part, q);
+          } catch (...) {
+            here.push_back(q);
+          }
+        }
         run_part(0);
+        for (const int q : here) run_part(q);
         for (auto &t : workers) t.join();
         for (int q = 0; q < parts; ++q)
           if (sts[q]) {
