@@ -166,8 +166,9 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // outside the block) and D^-1 for block blk of frequency f.
 //
 // One workgroup per frequency; lane r of every wave holds row r, and wave g
-// holds the columns j = g (mod 4) in registers (16 slots), so a column step
-// costs each lane 16 complex multiply-adds and one barrier.  Pivot columns
+// of the eight holds the columns j = g (mod 8) in registers (8 slots), so a
+// column step costs each lane 8 complex multiply-adds and one barrier, with
+// two waves per SIMD to cover each other's dependency stalls.  Pivot columns
 // reach the other waves through a two-entry LDS ring.  Linv is built in the
 // same pass: the row operations of the elimination (row r -= l_rc row c) are
 // applied to an identity X, which becomes L11^-1.  Slot j holds A[r][j]
@@ -180,6 +181,11 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // The working type C is the factor's type: FP64 for the FP64 factorization,
 // FP32 for the FP32 one (whose products elsewhere carry split-bf16 rounding
 // far above FP32's, and which refinement in FP64 corrects).
+
+// Waves of the diagonal-block kernel (two per SIMD) and the columns each
+// holds in registers.
+constexpr int kDiagWaves = 8;
+constexpr int kDiagSlots = kB / kDiagWaves;
 
 template <typename C>
 struct Zc {
@@ -211,7 +217,7 @@ __device__ __forceinline__ float readlane_c(float v, int lane) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__restrict__ im,
+__global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ re, T *__restrict__ im,
                                                    int64_t ld, int64_t mat_stride, int k0, int w,
                                                    int blk, int nblk, T *__restrict__ linvt_re,
                                                    T *__restrict__ linvt_im, T *__restrict__ dinv_re,
@@ -227,10 +233,10 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
   // L11 by columns, D and D^-1: written to memory after the loop, so that
   // no global store is outstanding at the barriers of the column steps
   __shared__ C lsr[kB][kB], lsi[kB][kB], dsr[kB], dsi[kB], vsr[kB], vsi[kB];
-  C mr[16], mi[16];
+  C mr[kDiagSlots], mi[kDiagSlots];
 #pragma unroll
-  for (int js = 0; js < 16; ++js) {
-    const int j = 4 * js + g;
+  for (int js = 0; js < kDiagSlots; ++js) {
+    const int j = kDiagWaves * js + g;
     const bool in = r < w && j <= r;
     const int64_t a = static_cast<int64_t>(k0 + min(j, rc)) * ld + k0 + rc;
     const C vr = Re[a], vi = Im[a];
@@ -263,16 +269,16 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
     // Written without branches on the slot's state (selects): a branch here
     // makes the compiler copy the register arrays through every arm.
     // All reads of the ring entry before its other entry is written.
-    C bcr[16], bci[16];
+    C bcr[kDiagSlots], bci[kDiagSlots];
 #pragma unroll
-    for (int js = 0; js < 16; ++js) {
-      bcr[js] = cbr[b][4 * js + g];
-      bci[js] = cbi[b][4 * js + g];
+    for (int js = 0; js < kDiagSlots; ++js) {
+      bcr[js] = cbr[b][kDiagWaves * js + g];
+      bci[js] = cbi[b][kDiagWaves * js + g];
     }
     const C delta = r == c ? C(1) : C(0);
 #pragma unroll
-    for (int js = 0; js < 16; ++js) {
-      const int j = 4 * js + g;
+    for (int js = 0; js < kDiagSlots; ++js) {
+      const int j = kDiagWaves * js + g;
       const bool from_ring = j > c;
       const C xcr = readlane_c(mr[js], c), xci = readlane_c(mi[js], c);
       const Zc<C> pr = zmulc(l, Zc<C>{from_ring ? bcr[js] : xcr, from_ring ? bci[js] : xci});
@@ -288,8 +294,8 @@ __global__ void __launch_bounds__(256) diag_kernel(T *__restrict__ re, T *__rest
   }
   const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
 #pragma unroll
-  for (int js = 0; js < 16; ++js) {
-    const int j = 4 * js + g;
+  for (int js = 0; js < kDiagSlots; ++js) {
+    const int j = kDiagWaves * js + g;
     const bool in = r < w && j < w;
     linvt_re[lbase + j * kB + r] = in ? mr[js] : T(0);
     linvt_im[lbase + j * kB + r] = in ? mi[js] : T(0);
@@ -1365,7 +1371,7 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
           if (st) return st;
         }
       }
-      diag_kernel<T><<<K, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
+      diag_kernel<T><<<K, 64 * kDiagWaves, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
                                             F.lt_re, F.lt_im, F.di_re, F.di_im, d_singular);
       FH_HIP_TRY(hipGetLastError());
       const int rows = L.N - (k0 + w);
