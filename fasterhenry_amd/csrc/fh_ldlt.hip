@@ -996,6 +996,7 @@ __global__ void __launch_bounds__(256) bwd_diag_kernel(
       const int i = tid;
       double ur = zsr[i], ui = zsi[i];
       const int64_t base = static_cast<int64_t>(B0 + i) * ld + B0 + s0;
+#pragma unroll 8
       for (int cc = 0; cc < sw; ++cc) {
         const double lr = static_cast<double>(Re[base + cc]), li = static_cast<double>(Im[base + cc]);
         ur -= lr * xsr[cc] - li * xsi[cc];
