@@ -32,7 +32,8 @@ struct DenseSolveInputs {
   bool tight_x = false;
 };
 
-// Grow-only device workspace.
+// Grow-only device workspace (through device_alloc, so that a batch
+// worker's block cache takes it back without a hipFree between problems).
 struct LdltWorkspace {
   char *base = nullptr;
   size_t bytes = 0;
@@ -41,18 +42,15 @@ struct LdltWorkspace {
   LdltWorkspace &operator=(const LdltWorkspace &) = delete;
   ~LdltWorkspace() { release(); }
   void release() {
-    if (base) (void)hipFree(base);
+    if (base) device_free(base);
     base = nullptr;
     bytes = 0;
   }
   fh_status reserve(size_t n) {
     if (n <= bytes) return FH_OK;
     release();
-    if (hipMalloc(&base, n) != hipSuccess) {
-      base = nullptr;
-      (void)hipGetLastError();
-      return set_error(FH_E_NO_MEMORY, "device workspace of " + std::to_string(n) + " bytes");
-    }
+    base = static_cast<char *>(device_alloc(n));
+    if (!base) return set_error(FH_E_NO_MEMORY, "device workspace of " + std::to_string(n) + " bytes");
     bytes = n;
     return FH_OK;
   }

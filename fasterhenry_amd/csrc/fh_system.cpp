@@ -978,7 +978,13 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
     if (stream) (void)hipStreamDestroy(stream);
   };
   std::vector<std::thread> pool;
-  for (size_t w = 1; w < workers; ++w) pool.emplace_back(worker);
+  for (size_t w = 1; w < workers; ++w) {
+    try {
+      pool.emplace_back(worker);
+    } catch (...) {
+      break;  // the workers started (this thread at least) take every problem
+    }
+  }
   worker();
   for (auto &t : pool) t.join();
   for (size_t b = 0; b < count; ++b)
