@@ -915,7 +915,9 @@ __global__ void moving_kernel(const fh_filament *__restrict__ T, size_t total, u
 // (blockIdx.y).  Light point quadrature is differentiated by the pair's
 // thread; the other methods are queued once (by the q = 0 workgroups) as
 // (f | kind << 30, g) for one wave per pair and direction.
-__global__ void __launch_bounds__(256) dtile_kernel(const fh_filament *__restrict__ F, int n, int nt,
+// Three waves per SIMD (168 registers, some spilled): 462 -> 271 ms on the
+// 6156-filament spiral against two; four spill more and measure 400 ms.
+__global__ void __launch_bounds__(256, 3) dtile_kernel(const fh_filament *__restrict__ F, int n, int nt,
                                                     const fh_filament *__restrict__ T,
                                                     const uint8_t *__restrict__ moving, Weights W, int nout,
                                                     double2 *__restrict__ partial, uint2 *__restrict__ queue,

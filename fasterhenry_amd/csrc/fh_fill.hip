@@ -777,7 +777,10 @@ __device__ __forceinline__ bool mask_keeps(const MaskArgs &m, int i, int j) {
   return m.table[flat] != 0;
 }
 
-__global__ void __launch_bounds__(256) fill_tile_kernel(const fh_filament *__restrict__ F, int n,
+// Four waves per SIMD (128 registers, a few spilled): the FP64 division and
+// square-root chains of the inline quadratures need the other waves to
+// cover them (12.0 -> 6.0 ms per fill of the batch problems against two).
+__global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__restrict__ F, int n,
                                                         int nt, double *__restrict__ L,
                                                         int8_t *__restrict__ methods,
                                                         uint2 *__restrict__ queue,
@@ -973,7 +976,7 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
 // Pair list: out[k] = partial inductance of filaments pairs[k] = (i, j).
 // Cheap pairs are finished by one thread each; the rest are queued (pair
 // index | kind << 30) for one wavefront each, as in the tile fill.
-__global__ void __launch_bounds__(256) pairs_kernel(const fh_filament *__restrict__ F, int n,
+__global__ void __launch_bounds__(256, 4) pairs_kernel(const fh_filament *__restrict__ F, int n,
                                                     const uint2 *__restrict__ pairs, size_t count,
                                                     double *__restrict__ out,
                                                     uint2 *__restrict__ queue,
