@@ -335,8 +335,8 @@ def model_pack(problem):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bars", type=int, default=49)
     ap.add_argument("--freqs", type=int, default=41)
     ap.add_argument("--precision", default="fp32_refined", choices=["fp64", "fp32_refined"])
