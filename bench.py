@@ -33,6 +33,12 @@ PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 # reading for a plain 512 MiB device copy (bench.py --copy-mib 512).
 UPDATE_TRAFFIC_BYTES = 885.6e6 + 65.2e6
 
+# HBM bytes per pFFT operator product (V = 2) on meander(157): the sum over
+# the product's kernels (spread, line passes, column DFTs, gather, near
+# field) of rocprofv3 FETCH_SIZE (4 630 MB) and WRITE_SIZE (2 359 MB) per
+# product, scaled as above.
+PFFT_TRAFFIC_BYTES = 4630.5e6 + 2358.9e6
+
 
 def cpu_model() -> str:
     try:
@@ -146,7 +152,7 @@ def matrix_free(steps: int, warmup: int, dense_check=None) -> dict:
                    "gmres_iterations_max": infos[-1]["gmres_iterations_max"],
                    "line_group": st["line_group"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": PFFT_TRAFFIC_BYTES,
                      "kernel": "pFFT operator product on the real and imaginary branch "
                                "currents (spread, 5 pruned FFT passes, gather, near field)",
                      "bytes_per_apply": traffic, "avg_apply_ms": apply_ms, "applies": applies},
