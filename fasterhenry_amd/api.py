@@ -19,6 +19,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import math
+import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -54,15 +55,32 @@ class Counts:
     ports: int
 
 
+def host_threads() -> int:
+    """Host threads the library's parallel host work may use (the pFFT
+    near-pair search caps its pool at 16): the ``threads`` of ``Timing``."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 @dataclass
 class Timing:
-    """result.rs:129-140 (threads is replaced by the device name)."""
+    """result.rs:129-140.  ``device`` is an extra key naming the GPU; a
+    reader of the reference layout (serde, which ignores unknown keys here)
+    sees the reference's five fields."""
 
-    device: str
+    threads: int
     inductance_s: float
     assembly_s: float
     solve_s: float
     total_s: float
+    device: str = "gfx950"
+
+    _FIELDS = ("threads", "inductance_s", "assembly_s", "solve_s", "total_s")
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k in self._FIELDS}
+        d["threads"] = int(d["threads"])
+        d["device"] = self.device
+        return d
 
 
 @dataclass
@@ -86,8 +104,15 @@ class SweepResult:
         omega = 2.0 * math.pi * self.frequencies_hz[index]
         return None if omega == 0.0 else self.impedance_ohm[index].imag / omega
 
+    def without_timing(self) -> "SweepResult":
+        """The deterministic part only (result.rs SweepResult::without_timing)."""
+        return SweepResult(list(self.ports), list(self.frequencies_hz), self.impedance_ohm.copy(),
+                           self.counts, None, self.schema_version, dict(self.extra))
+
     def to_json(self) -> str:
-        """The reference's JSON layout (result.rs:1-37)."""
+        """The reference's JSON layout (result.rs:1-37): ``impedance_ohm[k][i][j]``
+        as ``{"re", "im"}`` objects, rows following ``ports``; timing only when
+        present.  Every number must be finite (result.rs:32)."""
         doc = {
             "schema_version": self.schema_version,
             "ports": [{"positive": int(p.positive), "negative": int(p.negative),
@@ -97,11 +122,34 @@ class SweepResult:
                                for row in m] for m in self.impedance_ohm],
             "provenance": {
                 "fasterhenry_version": f"fasterhenry_amd {VERSION}",
-                "counts": self.counts.__dict__,
-                **({"timing": self.timing.__dict__} if self.timing else {}),
+                "counts": {k: int(v) for k, v in self.counts.__dict__.items()},
+                **({"timing": self.timing.as_dict()} if self.timing else {}),
             },
         }
-        return json.dumps(doc)
+        return json.dumps(doc, allow_nan=False)
+
+    @classmethod
+    def from_json(cls, text: str) -> "SweepResult":
+        """Read the schema-1 layout, as written here or by the reference."""
+        doc = json.loads(text)
+        if doc.get("schema_version") != cls.SCHEMA_VERSION:
+            raise InvalidArgument(f"unsupported schema_version {doc.get('schema_version')!r}")
+        ports = [Port(int(p["positive"]), int(p["negative"]), p.get("name")) for p in doc["ports"]]
+        freqs = [float(f) for f in doc["frequencies_hz"]]
+        mats = []
+        for rows in doc["impedance_ohm"]:
+            if any(len(r) != len(rows) for r in rows):
+                raise InvalidArgument("impedance matrix is not square")
+            mats.append([[complex(e["re"], e["im"]) for e in r] for r in rows])
+        z = np.array(mats, dtype=np.complex128).reshape(len(mats), len(ports), len(ports))
+        prov = doc["provenance"]
+        counts = Counts(**{k: int(prov["counts"][k]) for k in Counts.__dataclass_fields__})
+        t = prov.get("timing")
+        timing = None
+        if t is not None:
+            timing = Timing(int(t["threads"]), float(t["inductance_s"]), float(t["assembly_s"]),
+                            float(t["solve_s"]), float(t["total_s"]), str(t.get("device", "")))
+        return cls(ports, freqs, z, counts, timing, int(doc["schema_version"]))
 
     def to_zc_mat(self) -> bytes:
         """MAT v4 bytes of ``freqs`` and ``Zc_1 .. Zc_K`` (fasterhenry-cli/src/mat.rs:31-95)."""
@@ -373,7 +421,7 @@ class MeshSystem:
     def sweep(self, frequencies) -> SweepResult:
         """Z at every frequency (solve.rs:881-894)."""
         z, info = self.sweep_raw(frequencies)
-        timing = Timing("gfx950", info.inductance_s, info.assembly_s, info.solve_s,
+        timing = Timing(host_threads(), info.inductance_s, info.assembly_s, info.solve_s,
                         info.assembly_s + info.solve_s)
         return SweepResult(self.ports, [float(x) for x in np.asarray(frequencies, dtype=float)], z,
                            self._counts, timing,
@@ -520,7 +568,7 @@ class IterativeSystem:
     def sweep(self, frequencies) -> SweepResult:
         """Z at every frequency (iterative.rs:387-404)."""
         z, info = self.sweep_raw(frequencies)
-        timing = Timing("gfx950", info.inductance_s, info.assembly_s, info.solve_s,
+        timing = Timing(host_threads(), info.inductance_s, info.assembly_s, info.solve_s,
                         info.assembly_s + info.solve_s)
         return SweepResult(self.ports, [float(x) for x in np.asarray(frequencies, dtype=float)], z,
                            self._counts, timing,

@@ -111,6 +111,10 @@ size_t parse_count(const std::string &text, const std::string &what, size_t line
     throw DeckError{line, "'" + text + "' is not a valid " + what + " (a whole number ≥ 1)"};
   const unsigned long long v = std::strtoull(text.c_str() + i, nullptr, 10);
   if (v < 1) throw DeckError{line, what + " must be ≥ 1, got " + std::to_string(v)};
+  // A segment's grid holds 32-bit counts; anything larger could never be
+  // discretized (2^32 filaments of 152 bytes), so refuse it here, by line.
+  if (v > 0xFFFFFFFFull)
+    throw DeckError{line, what + " = " + text + " is too large (at most 4294967295 filaments per axis)"};
   return static_cast<size_t>(v);
 }
 

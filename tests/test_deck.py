@@ -111,8 +111,15 @@ def test_equiv_joins_nodes_and_unreferenced_nodes_are_dropped():
     (lambda t: t.replace(".units mm", ".units furlong"), 1),
     (lambda t: t.replace("ndec=0.5", "ndec=0"), 7),
     (lambda t: "+ continued\n" + t, 1),
+    (lambda t: t.replace("E1 N1 N2", "E1 N1 N2 nwinc=4294967297"), 5),
+    (lambda t: t.replace("E1 N1 N2", "E1 N1 N2 nhinc=18446744073709551615"), 5),
+    (lambda t: t.replace("E1 N1 N2", "E1 N1 N2 nwinc=123456789012345678901"), 5),
+    (lambda t: t.replace("E1 N1 N2", "E1 N1 N2 nwinc=0"), 5),
+    (lambda t: t.replace("E1 N1 N2", "E1 N1 N2 nwinc=2.5"), 5),
 ])
 def test_rejected_decks_name_their_line(change, line):
+    """Counts beyond a segment grid's 32 bits are refused by line instead of
+    wrapping (nwinc = 2^32 + 1 must not become 1 filament)."""
     with pytest.raises(DeckError) as e:
         deck.parse(change(BASE))
     if line is not None:
