@@ -70,6 +70,28 @@ def cpu_baseline(system, omega_hz: float, threads: int) -> dict:
     return {"seconds": dt, "z": z}
 
 
+def cpu_end_to_end(threads: int) -> dict:
+    """The reference's published CPU timing (docs/benchmarks.md:127-166):
+    assemble + sweep at 1 MHz of the square meander at 2 x 2, here through
+    the oracle at 960 and 2 400 filaments, on 1 thread and on `threads`."""
+    import oracle
+    from fasterhenry_amd import fixtures, model
+
+    out = {"timed": "assemble + sweep(&[1 MHz]) of meander(bars) 2x2, oracle",
+           "reference_1_thread_s": {"960": 0.84, "2400": 4.15},
+           "reference_host": "AWS EC2 8 vCPU Xeon-class, 1 thread (different hardware)"}
+    for bars in (15, 24):
+        pk = model.pack(*fixtures.meander_problem(bars))
+        row = {}
+        for t in (1, threads):
+            t0 = time.perf_counter()
+            oracle.System(pk, t).sweep([1e6])
+            row[f"{t}_thread_s" if t == 1 else f"{t}_threads_s"] = time.perf_counter() - t0
+        out[str(pk.filament_count)] = row
+        print(f"# cpu end to end {pk.filament_count} filaments: {row}", flush=True)
+    return out
+
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -421,25 +443,47 @@ def main() -> int:
     upd_unit_flops = sum(i["update_kernel_unit_flops"] for i in infos)
     unit_kind = "fp64" if args.precision == "fp64" else "fp32"
     # The roofline is priced on the units the dominant kernel runs on: FP64 or
-    # FP32 matrix cores, or the bf16 matrix cores for split-bf16 updates (three
-    # bf16 products per FP32-equivalent product).
+    # FP32 matrix cores, or the bf16 matrix cores for split-bf16 updates.
+    # `achieved` counts the algorithmic flops only (8 real flops per complex
+    # multiply-add of the trailing update, the FP32-equivalent work); the
+    # three bf16 products that carry each split product are reported apart
+    # as `matrix_core_tflops`, so frac <= 1/3 for split-bf16.
     arith = infos[-1]["factor_arith"]
     peak_kind = {0: "fp64", 1: "fp32", 2: "bf16"}[arith]
     peak = PEAKS_TFLOPS[peak_kind]
-    achieved = upd_unit_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
-    achieved_equiv = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
+    unit_rate = upd_unit_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
 
     extra = {}
     m = counts.internal_meshes
     if args.precision != "fp64" and not args.no_compare:
+        # the plain FP64 form on the same resident system, with the same
+        # warm-up and step count (its workspace grows during the warm-up)
         system.set_precision("fp64")
+        for w in range(args.warmup):
+            system.sweep_raw(freqs)
+        system.set_profiling(True)
+        infos64 = []
         t1 = time.perf_counter()
-        z64, _ = system.sweep_raw(freqs)
-        t64 = time.perf_counter() - t1
+        for k in range(args.steps):
+            z64, info64 = system.sweep_raw(freqs)
+            infos64.append(info64.as_dict())
+            print(f"# fp64 step {k + 1}/{args.steps}: {info64.solve_s:.3f} s", flush=True)
+        t64 = (time.perf_counter() - t1) / args.steps
+        system.set_profiling(False)
         system.set_precision(args.precision)
         err = max(np.linalg.norm(z[k] - z64[k]) / np.linalg.norm(z64[k]) for k in range(len(freqs)))
+        ms64 = sum(i["update_kernel_ms"] for i in infos64)
+        n64 = sum(i["update_kernel_launches"] for i in infos64)
+        fl64 = sum(i["update_kernel_flops"] for i in infos64)
+        a64 = fl64 / (ms64 * 1e-3) / 1e12 if ms64 > 0 else 0.0
         extra = {"z_rel_err_vs_fp64": err, "fp64_solves_per_s": args.freqs / t64,
-                 "speedup_vs_fp64": (args.freqs / t64) and value / (args.freqs / t64)}
+                 "speedup_vs_fp64": value / (args.freqs / t64),
+                 "fp64_form": {"solves_per_s": args.freqs / t64, "ms_per_step": t64 * 1e3,
+                               "steps": args.steps, "warmup": args.warmup,
+                               "update_kernel": "right-looking trailing update (fp64 matrix cores)",
+                               "update_achieved_tflops": a64, "update_frac": a64 / PEAKS_TFLOPS["fp64"],
+                               "update_launches": n64, "update_avg_launch_ms": ms64 / max(n64, 1)}}
 
     cpu = None
     if not args.no_cpu:
@@ -452,7 +496,10 @@ def main() -> int:
                "sample": f"one frequency point ({freqs[len(freqs) // 2]:.4g} Hz) of the same "
                          f"m={m} system: the oracle's blocked complex LU (PANEL 32, partial "
                          f"pivoting) + Schur, OpenMP on {threads} threads of {cpu_model()}",
-               "z_rel_err_gpu_vs_cpu": float(np.linalg.norm(zg - res["z"]) / np.linalg.norm(res["z"]))}
+               "z_rel_err_gpu_vs_cpu": float(np.linalg.norm(zg - res["z"]) / np.linalg.norm(res["z"])),
+               "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+               "cpu_model": cpu_model()}
+        cpu["end_to_end"] = cpu_end_to_end(threads)
 
     out = {
         "metric": "freq-point solves/s @9.8k filaments: refined FP32 vs FP64 LU, Z err ≤1e-10",
@@ -478,19 +525,23 @@ def main() -> int:
                    "refine_change_max": infos[-1]["refine_change_max"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
+                     "flops": "algorithmic: 8 real flops per complex multiply-add of the update "
+                              "(FP32-equivalent), (4/3) m^3 per frequency over all launches",
                      "traffic": UPDATE_TRAFFIC_BYTES if (args.bars, args.freqs, args.precision,
                                                          args.sweep_parts) == (49, 41, "fp32_refined", 2)
                      else None,
                      "kernel": "outer trailing update of the L D L^T factorization "
                                f"({peak_kind} matrix cores)",
-                     "achieved_equivalent_tflops": achieved_equiv,
+                     "matrix_core_tflops": unit_rate,
+                     "matrix_core_frac": unit_rate / peak,
                      "launches": upd_n, "avg_launch_ms": upd_ms / max(upd_n, 1),
+                     "flops_per_launch": upd_flops / max(upd_n, 1),
                      "streams": args.sweep_parts if args.precision != "fp64" else 1,
                      # with the parts' launches overlapping, a launch's own
                      # duration stretches; this is every update flop of the
                      # timed steps over their whole wall time
-                     "device_achieved": upd_unit_flops / elapsed / 1e12,
-                     "device_frac": upd_unit_flops / elapsed / 1e12 / peak},
+                     "device_achieved": upd_flops / elapsed / 1e12,
+                     "device_frac": upd_flops / elapsed / 1e12 / peak},
         "cpu_baseline": cpu,
         **extra,
     }
