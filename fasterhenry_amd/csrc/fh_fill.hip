@@ -1356,6 +1356,109 @@ fh_status fill_pairs(const fh_filament *d_F, size_t n, const uint2 *d_pairs, siz
   return FH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The fill without allocation: the deferred-pair queue, its count and the
+// error key live in `scratch`, a fixed device array (each GPU holds its own
+// copy until the process ends), sized for kNoAllocMaxFilaments filaments
+// (every pair of the upper triangle queued).  The memo is not used here (it
+// needs sort buffers); the deferred kernel evaluates each queued pair itself,
+// which gives the same bits.
+
+}  // namespace fh
+
+namespace {
+constexpr size_t kNoAllocMaxFilaments = 4000;
+constexpr size_t kNoAllocQueueBytes =
+    kNoAllocMaxFilaments * (kNoAllocMaxFilaments + 1) / 2 * sizeof(uint2);
+constexpr size_t kNoAllocScratchBytes = kNoAllocQueueBytes + 256;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace fh {
+namespace {
+
+__global__ void noalloc_clear_kernel(unsigned int *qcount, unsigned long long *err) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *qcount = 0u;
+    *err = ~0ULL;
+  }
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the symbol's host-side name is not a device address).
+fh_status noalloc_scratch(unsigned char **out) {
+  static std::mutex mu;
+  static std::vector<unsigned char *> per_device;
+  int dev = 0;
+  FH_HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (static_cast<int>(per_device.size()) <= dev) per_device.resize(dev + 1, nullptr);
+  if (!per_device[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(FH_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                     hipGetErrorString(e));
+    per_device[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = per_device[dev];
+  return FH_OK;
+}
+
+}  // namespace
+
+size_t fill_noalloc_max_filaments() { return kNoAllocMaxFilaments; }
+
+fh_status fill_inductance_noalloc(const fh_filament *d_F, size_t n, double *d_L, int8_t *d_methods,
+                                  hipStream_t stream) {
+  if (n > kNoAllocMaxFilaments)
+    return set_error(FH_E_INVALID_ARGUMENT,
+                     "fh_partial_inductance_noalloc: " + std::to_string(n) +
+                         " filaments exceed the limit of its fixed scratch buffer (" +
+                         std::to_string(kNoAllocMaxFilaments) + "); use fh_partial_inductance",
+                     static_cast<int64_t>(n), static_cast<int64_t>(kNoAllocMaxFilaments));
+  // set-up on this device: the scratch address and the constant tables
+  unsigned char *base = nullptr;
+  fh_status st = noalloc_scratch(&base);
+  if (st) return st;
+  if ((st = upload_tables())) return st;
+  if (n == 0) return FH_OK;
+  uint2 *queue = reinterpret_cast<uint2 *>(base);
+  unsigned int *qcount = reinterpret_cast<unsigned int *>(base + kNoAllocQueueBytes);
+  unsigned long long *err = reinterpret_cast<unsigned long long *>(base + kNoAllocQueueBytes + 8);
+  const int nt = static_cast<int>((n + kTile - 1) / kTile);
+  const long long tiles = static_cast<long long>(nt) * (nt + 1) / 2;
+  noalloc_clear_kernel<<<1, 64, 0, stream>>>(qcount, err);
+  FH_HIP_TRY(hipGetLastError());
+  const MaskArgs mask{nullptr, 0, 0, nullptr};
+  fill_tile_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
+      d_F, static_cast<int>(n), nt, d_L, d_methods, queue, qcount, err, mask);
+  FH_HIP_TRY(hipGetLastError());
+  int dev = 0, cus = 256;
+  FH_HIP_TRY(hipGetDevice(&dev));
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  fill_deferred_kernel<<<static_cast<unsigned>(cus) * 16, 64, 0, stream>>>(
+      d_F, static_cast<int>(n), d_L, d_methods, queue, qcount, err, 0);
+  FH_HIP_TRY(hipGetLastError());
+  unsigned long long key = 0;
+  FH_HIP_TRY(hipMemcpyAsync(&key, err, sizeof(key), hipMemcpyDeviceToHost, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));
+  if (key != ~0ULL) {
+    const int code = static_cast<int>(key & 3u);
+    const unsigned long long pair = key >> 2;
+    const int64_t i = static_cast<int64_t>(pair / n), j = static_cast<int64_t>(pair % n);
+    return set_error(FH_E_KERNEL,
+                     code == 1 ? "overlapping parallel filaments " + std::to_string(i) + " and " +
+                                     std::to_string(j) +
+                                     " have misaligned cross-sections and coincident sample lines"
+                               : "partial inductance of filaments " + std::to_string(i) + " and " +
+                                     std::to_string(j) + " is not finite",
+                     i, j, 0.0, code);
+  }
+  return FH_OK;
+}
+
 void gauss_rule_host(int n, double *nodes, double *weights) { host_rule(n, nodes, weights); }
 
 fh_status device_math(const double *x, size_t n, double *log_out, double *atan_out) {

@@ -123,6 +123,13 @@ void congruence_resistance(const LoopBasis &basis, const std::vector<double> &r,
 
 // Device-side portable math check (fh_fill.hip).
 fh_status device_math(const double *x, size_t n, double *log_out, double *atan_out);
+// The fill with its deferred-pair queue in a fixed per-device scratch array
+// instead of allocated memory (fh_fill.hip); n at most
+// fill_noalloc_max_filaments().  Same values and methods as fill_inductance
+// without a mask.
+fh_status fill_inductance_noalloc(const fh_filament *d_filaments, size_t n, double *d_L,
+                                  int8_t *d_methods, hipStream_t stream);
+size_t fill_noalloc_max_filaments();
 // Partial inductances of the filament pairs d_pairs[k] = (i, j) into d_out[k]
 // (device arrays), bit-identical to the same entries of fill_inductance.
 fh_status fill_pairs(const fh_filament *d_filaments, size_t n, const uint2 *d_pairs, size_t count,

@@ -1036,6 +1036,16 @@ fh_status fh_partial_inductance_masked(const fh_filament *filaments, size_t n, c
   return FH_OK;
 }
 
+fh_status fh_partial_inductance_noalloc(const fh_filament *d_filaments, size_t n, double *d_L_out,
+                                        int8_t *d_methods_out, void *stream) {
+  clear_error();
+  if (n && (!d_filaments || !d_L_out)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  fh_status st = check_device();
+  if (st) return st;
+  return fill_inductance_noalloc(d_filaments, n, d_L_out, d_methods_out,
+                                 static_cast<hipStream_t>(stream));
+}
+
 fh_status fh_pfft_create(const fh_filament *filaments, size_t n, const fh_pfft_params *params,
                          fh_pfft **out) {
   clear_error();

@@ -34,7 +34,7 @@ SYMBOLS = [
     "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path", "fh_system_create_coupled",
     "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
     "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
-    "fh_system_set_sweep_parts",
+    "fh_system_set_sweep_parts", "fh_partial_inductance_noalloc",
 ]
 
 _ERRORS = {
@@ -168,6 +168,8 @@ def lib() -> C.CDLL:
                                C.POINTER(SweepInfo)]
         L.fh_partial_inductance.restype = st
         L.fh_partial_inductance.argtypes = [vp, sz, vp, vp]
+        L.fh_partial_inductance_noalloc.restype = st
+        L.fh_partial_inductance_noalloc.argtypes = [vp, sz, vp, vp, vp]
         L.fh_discretize.restype = st
         L.fh_discretize.argtypes = [C.POINTER(Problem), vp, sz, C.POINTER(sz)]
         L.fh_device_math.restype = st

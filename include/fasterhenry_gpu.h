@@ -431,6 +431,23 @@ void fh_deck_destroy(fh_deck *deck);
 fh_status fh_partial_inductance(const fh_filament *filaments, size_t n, double *L_out,
                                 int8_t *methods);
 
+/* The fill of fh_partial_inductance without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  d_filaments (n records), d_L_out
+ * (n x n doubles, row-major) and d_methods_out (n x n bytes, may be NULL)
+ * are device pointers the caller owns; stream is a hipStream_t (NULL: the
+ * null stream).  The deferred-pair queue lives in a fixed scratch array of
+ * each GPU (64 MB, held until the process ends), sized for
+ * FH_NOALLOC_MAX_FILAMENTS filaments; a larger n returns
+ * FH_E_INVALID_ARGUMENT.  Every call on a GPU shares that array, so calls
+ * that overlap on one GPU race: serialize them.  The values and methods are
+ * fh_partial_inductance's, bit for bit; the call synchronizes the stream
+ * before it returns (to report a kernel error). */
+#define FH_NOALLOC_MAX_FILAMENTS 4000
+fh_status fh_partial_inductance_noalloc(const fh_filament *d_filaments, size_t n, double *d_L_out,
+                                        int8_t *d_methods_out, void *stream);
+
 /* discretize_geometry (solve.rs:947-975) on the host: n_out receives the
  * filament count; out (capacity entries) may be NULL to query the count. */
 fh_status fh_discretize(const fh_problem *problem, fh_filament *out, size_t capacity,
