@@ -400,6 +400,11 @@ class MeshSystem:
     def set_profiling(self, enable: bool) -> None:
         native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
 
+    def set_refinement(self, stages: int) -> None:
+        """The refinement stages the refined form may use (a mask of
+        native.FH_REFINE_*; fh_system_set_refinement)."""
+        native.check(native.lib().fh_system_set_refinement(self._h, int(stages)))
+
     def set_sweep_parts(self, parts: int) -> None:
         """Refined dense sweeps: factorize a chunk of frequencies as 1 to 4
         groups on as many streams (default 2; a group has at least 4
@@ -427,7 +432,8 @@ class MeshSystem:
                            self._counts, timing,
                            extra={"refine_steps_max": info.refine_steps_max,
                                   "refine_change_max": info.refine_change_max,
-                                  "factor_arith": info.factor_arith})
+                                  "factor_arith": info.factor_arith,
+                                  "refine_stage": info.refine_stage})
 
     def impedance(self, frequency: float) -> np.ndarray:
         """Z at one frequency (solve.rs:806-830)."""

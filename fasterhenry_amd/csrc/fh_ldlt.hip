@@ -47,6 +47,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <type_traits>
 
@@ -61,6 +62,9 @@ constexpr int kKC = 16;       // k-chunk staged in LDS
 constexpr int kNbPlain = 128; // outer block, FP64 form
 constexpr int kNbRefined = 256;  // outer block, FP32 form
 constexpr int kMaxRefine = 30;
+constexpr int kMaxGmresRefine = 8;   // outer steps under GMRES-IR
+constexpr int kGmresRestart = 20;    // inner GMRES steps per outer step
+constexpr double kGmresTol = 1e-8;   // inner reduction of the preconditioned residual
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -1219,6 +1223,127 @@ __global__ void __launch_bounds__(256) norms_kernel(int m, const double *__restr
 }
 
 // ---------------------------------------------------------------------------
+// GMRES-IR (Carson & Higham 2017-2018): the correction equation Zee d = r of
+// each refinement step solved by GMRES on M^-1 Zee d = M^-1 r, M the FP32
+// factorization, everything but M in FP64.  The V complex vectors of a batch
+// run in lockstep, each with its own Krylov basis and Hessenberg system (on
+// the host); vector v's real row is 2 v, its imaginary row 2 v + 1, as in X.
+
+// Y = (R + j w L) X given P = L X.
+__global__ void zee_apply_kernel(int m, int p, int V, const double *__restrict__ omegas,
+                                 const int64_t *__restrict__ rptr, const int32_t *__restrict__ rcol,
+                                 const double *__restrict__ rval, const double *__restrict__ X,
+                                 const double *__restrict__ P, double *__restrict__ Y) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(V) * m) return;
+  const int v = static_cast<int>(t / m), i = static_cast<int>(t % m);
+  const double w = omegas[v / p];
+  const double *xr = X + static_cast<int64_t>(2 * v) * m, *xi = xr + m;
+  double sr = 0.0, si = 0.0;
+  for (int64_t e = rptr[i]; e < rptr[i + 1]; ++e) {
+    sr += rval[e] * xr[rcol[e]];
+    si += rval[e] * xi[rcol[e]];
+  }
+  const double pr = P[static_cast<int64_t>(2 * v) * m + i], pim = P[static_cast<int64_t>(2 * v + 1) * m + i];
+  Y[static_cast<int64_t>(2 * v) * m + i] = sr - w * pim;
+  Y[static_cast<int64_t>(2 * v + 1) * m + i] = si + w * pr;
+}
+
+// h[v][i] = <b_i, w>_v = sum conj(b_i) w over vector v, for i < nb; one
+// block per (i, v), fixed-order reduction.  Basis vector i starts at
+// basis + i * bstride.
+__global__ void __launch_bounds__(256) cdot_kernel(int m, int hld, const double *__restrict__ basis,
+                                                   int64_t bstride, const double *__restrict__ W,
+                                                   double *__restrict__ h) {
+  const int i = blockIdx.x, v = blockIdx.y;
+  const double *br = basis + i * bstride + static_cast<int64_t>(2 * v) * m, *bi = br + m;
+  const double *wr = W + static_cast<int64_t>(2 * v) * m, *wi = wr + m;
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < m; t += blockDim.x) {
+    a += br[t] * wr[t] + bi[t] * wi[t];
+    b += br[t] * wi[t] - bi[t] * wr[t];
+  }
+  __shared__ double ra[256], rb[256];
+  ra[threadIdx.x] = a;
+  rb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s2) {
+      ra[threadIdx.x] += ra[threadIdx.x + s2];
+      rb[threadIdx.x] += rb[threadIdx.x + s2];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    h[2 * (static_cast<int64_t>(v) * hld + i)] = ra[0];
+    h[2 * (static_cast<int64_t>(v) * hld + i) + 1] = rb[0];
+  }
+}
+
+// W_v -= sum_{i < nb} c[v][i] b_i (sign = -1), or D_v = sum_{i < nb} c[v][i] b_i
+// (sign = +1, W ignored on input), c complex with leading dimension cld.
+__global__ void basis_combine_kernel(int m, int V, int nb, const double *__restrict__ basis,
+                                     int64_t bstride, const double *__restrict__ c, int cld,
+                                     double *__restrict__ W, int sign) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(V) * m) return;
+  const int v = static_cast<int>(t / m), k = static_cast<int>(t % m);
+  const int64_t o = static_cast<int64_t>(2 * v) * m + k;
+  double ar = 0.0, ai = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const double hr = c[2 * (static_cast<int64_t>(v) * cld + i)];
+    const double hi = c[2 * (static_cast<int64_t>(v) * cld + i) + 1];
+    const double br = basis[i * bstride + o], bi = basis[i * bstride + o + m];
+    ar += hr * br - hi * bi;
+    ai += hr * bi + hi * br;
+  }
+  if (sign < 0) {
+    W[o] -= ar;
+    W[o + m] -= ai;
+  } else {
+    W[o] = ar;
+    W[o + m] = ai;
+  }
+}
+
+// nrm[v] = |W_v|^2, fixed-order reduction.
+__global__ void __launch_bounds__(256) vnorm_kernel(int m, const double *__restrict__ W,
+                                                    double *__restrict__ nrm) {
+  const int v = blockIdx.x;
+  const double *a = W + static_cast<int64_t>(2 * v) * m;
+  double s2 = 0.0;
+  for (int t = threadIdx.x; t < 2 * m; t += blockDim.x) s2 += a[t] * a[t];
+  __shared__ double r[256];
+  r[threadIdx.x] = s2;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (static_cast<int>(threadIdx.x) < k) r[threadIdx.x] += r[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) nrm[v] = r[0];
+}
+
+// dst_v = W_v / sqrt(nrm[v]) (0 where the norm is 0 or not finite).
+__global__ void vscale_kernel(int m, int V, const double *__restrict__ W,
+                              const double *__restrict__ nrm, double *__restrict__ dst) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(2) * V * m) return;
+  const int v = static_cast<int>(t / (2 * static_cast<int64_t>(m)));
+  const double q = nrm[v];
+  const double inv = q > 0.0 && q < INFINITY ? 1.0 / __builtin_sqrt(q) : 0.0;
+  dst[t] = W[t] * inv;
+}
+
+// Zeroes the vectors of the frequencies flagged in `reset`.
+__global__ void reset_vectors_kernel(int m, int p, int V, const int *__restrict__ reset,
+                                     double *__restrict__ X) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(2) * V * m) return;
+  const int v = static_cast<int>(t / (2 * static_cast<int64_t>(m)));
+  if (reset[v / p]) X[t] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // Host drivers.
 
 struct Layout {
@@ -1444,7 +1569,10 @@ size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined) {
   in.p = p;
   if (!refined) return factor_bytes<double>(layout_of(in, kNbPlain), 1);
   const Layout L = layout_of(in, kNbRefined);
-  return factor_bytes<float>(L, 1) + static_cast<size_t>(p) * m * 2 * 20 * sizeof(double) +
+  // the FP64 vectors of the refinement: X, R, DX, P, W, up to 16 partial
+  // sums of L X and the GMRES-IR basis, 2 p rows of m each
+  return factor_bytes<float>(L, 1) +
+         static_cast<size_t>(p) * m * 2 * (5 + 16 + kGmresRestart + 1 + 2) * sizeof(double) +
          2 * static_cast<size_t>(split_blocks(L)) * 4096;
 }
 
@@ -1469,6 +1597,7 @@ fh_status ldlt_solve_batch(const DenseSolveInputs &in, const double *d_omegas, i
   out.refine_steps.assign(K, 0);
   out.refine_change.assign(K, 0.0);
   out.converged.assign(K, 1);
+  out.stage.assign(K, -1);
   return FH_OK;
 }
 
@@ -1483,10 +1612,16 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   const int itiles = (m + 63) / 64, qtiles = (Q + 63) / 64;
   const int lx_slices = std::max(1, std::min(16, 2048 / std::max(1, itiles * qtiles)));
   const int lx_slice_len = ((m + lx_slices - 1) / lx_slices + 31) / 32 * 32;
-  const size_t extra = (4 + static_cast<size_t>(lx_slices)) * vbytes +
-                       static_cast<size_t>(K) * p * p * sizeof(double) +
-                       2 * static_cast<size_t>(V) * sizeof(double) +
-                       static_cast<size_t>(K) * sizeof(int) + 256;
+  // X, R, DX, P, the lx slices, the GMRES basis and its work vector, then
+  // the small arrays
+  const int hld = kGmresRestart + 1;
+  const size_t small = static_cast<size_t>(K) * p * p * sizeof(double) +  // metric
+                       2 * static_cast<size_t>(V) * sizeof(double) +     // nrm
+                       static_cast<size_t>(V) * sizeof(double) +         // vnrm
+                       4 * static_cast<size_t>(V) * hld * sizeof(double) +  // h (2 passes)
+                       2 * static_cast<size_t>(V) * hld * sizeof(double) +  // y
+                       2 * static_cast<size_t>(K) * sizeof(int) + 256;      // sing, reset
+  const size_t extra = (5 + static_cast<size_t>(lx_slices) + hld) * vbytes + small;
   const size_t sbytes = static_cast<size_t>(split_blocks(L)) * 4096 * K;
   if ((st = ws.reserve(fbytes + extra + 2 * sbytes + 512))) return st;
   const Factor<float> F = carve<float>(ws.base, L, K);
@@ -1498,21 +1633,157 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     sb.l = reinterpret_cast<bf16x8 *>(sp);
     sb.w = reinterpret_cast<bf16x8 *>(sp + sbytes);
   }
+  const size_t n = static_cast<size_t>(Q) * m;
   char *cur = ws.base + ((fbytes + 255) / 256) * 256;
   double *X = reinterpret_cast<double *>(cur);
-  double *R = X + static_cast<size_t>(Q) * m;
-  double *DX = R + static_cast<size_t>(Q) * m;
-  double *P = DX + static_cast<size_t>(Q) * m;
-  double *metric = P + static_cast<size_t>(Q) * m;
-  double *lx_part = metric + static_cast<size_t>(K) * p * p;
-  double *d_nrm = lx_part + static_cast<size_t>(lx_slices) * Q * m;
-  int *d_sing = reinterpret_cast<int *>(d_nrm + 2 * static_cast<size_t>(V));
-  const size_t n = static_cast<size_t>(Q) * m;
+  double *R = X + n;
+  double *DX = R + n;
+  double *P = DX + n;
+  double *Wv = P + n;
+  double *lx_part = Wv + n;
+  double *basis = lx_part + static_cast<size_t>(lx_slices) * n;
+  double *metric = basis + static_cast<size_t>(hld) * n;
+  double *d_nrm = metric + static_cast<size_t>(K) * p * p;
+  double *d_vnrm = d_nrm + 2 * static_cast<size_t>(V);
+  double *d_h = d_vnrm + V;
+  double *d_y = d_h + 4 * static_cast<size_t>(V) * hld;
+  int *d_sing = reinterpret_cast<int *>(d_y + 2 * static_cast<size_t>(V) * hld);
+  int *d_reset = d_sing + K;
   const unsigned vec_blocks = static_cast<unsigned>((n + 255) / 256);
   const unsigned res_blocks = static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256);
   const int64_t lds = L.N;
   std::vector<double> h_metric(static_cast<size_t>(K) * p * p), h_nrm(2 * static_cast<size_t>(V));
   std::vector<double> h_z(2 * static_cast<size_t>(K) * p * p);
+  std::vector<int> h_sing(K, 0), h_reset(K, 0);
+
+  // P = L X, then R = b - Zee X.
+  auto residual = [&]() -> fh_status {
+    lx_kernel<<<dim3(itiles, qtiles, lx_slices), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X, lx_part,
+                                                                   lx_slice_len);
+    lx_reduce_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), lx_slices, lx_part, P);
+    residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
+                                                    in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
+    FH_HIP_TRY(hipGetLastError());
+    return FH_OK;
+  };
+
+  // DX ~= Zee^-1 R by GMRES on M^-1 Zee d = M^-1 R from d = 0: one cycle of
+  // up to kGmresRestart steps, stopped per vector when the preconditioned
+  // residual falls below kGmresTol of its start.  P and Wv are scratch.
+  auto gmres_correction = [&](int &steps_out) -> fh_status {
+    typedef std::complex<double> cd;
+    const int R1 = kGmresRestart;
+    std::vector<double> beta(V), hh(4 * static_cast<size_t>(V) * hld), vn(V);
+    std::vector<cd> H(static_cast<size_t>(V) * (R1 + 1) * R1), g(static_cast<size_t>(V) * (R1 + 1));
+    std::vector<double> cs(static_cast<size_t>(V) * R1);
+    std::vector<cd> sn(static_cast<size_t>(V) * R1);
+    std::vector<int> done(V, 0), used(V, 0);
+    // b_0 = M^-1 R / beta
+    FH_HIP_TRY(hipMemcpyAsync(Wv, R, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    fh_status s2 = factor_solve<float>(L, F, V, Wv, P, stream);
+    if (s2) return s2;
+    vnorm_kernel<<<V, 256, 0, stream>>>(m, Wv, d_vnrm);
+    vscale_kernel<<<vec_blocks, 256, 0, stream>>>(m, V, Wv, d_vnrm, basis);
+    FH_HIP_TRY(hipGetLastError());
+    FH_HIP_TRY(hipMemcpyAsync(vn.data(), d_vnrm, V * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FH_HIP_TRY(hipStreamSynchronize(stream));
+    for (int v = 0; v < V; ++v) {
+      beta[v] = std::sqrt(vn[v]);
+      g[static_cast<size_t>(v) * (R1 + 1)] = cd(beta[v], 0.0);
+      if (!(beta[v] > 0.0) || !std::isfinite(beta[v])) done[v] = 1;
+    }
+    int j = 0;
+    for (; j < R1; ++j) {
+      bool all = true;
+      for (int v = 0; v < V; ++v) all = all && done[v];
+      if (all) break;
+      const double *bj = basis + static_cast<size_t>(j) * n;
+      // W = M^-1 Zee b_j
+      lx_kernel<<<dim3(itiles, qtiles, lx_slices), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, bj, lx_part,
+                                                                     lx_slice_len);
+      lx_reduce_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), lx_slices, lx_part, P);
+      zee_apply_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
+                                                       in.d_ree_val, bj, P, Wv);
+      FH_HIP_TRY(hipGetLastError());
+      if ((s2 = factor_solve<float>(L, F, V, Wv, P, stream))) return s2;
+      // classical Gram-Schmidt, twice
+      for (int pass = 0; pass < 2; ++pass) {
+        double *hp = d_h + static_cast<size_t>(pass) * 2 * V * hld;
+        cdot_kernel<<<dim3(j + 1, V), 256, 0, stream>>>(m, hld, basis, static_cast<int64_t>(n), Wv, hp);
+        basis_combine_kernel<<<res_blocks, 256, 0, stream>>>(m, V, j + 1, basis, static_cast<int64_t>(n),
+                                                             hp, hld, Wv, -1);
+      }
+      vnorm_kernel<<<V, 256, 0, stream>>>(m, Wv, d_vnrm);
+      vscale_kernel<<<vec_blocks, 256, 0, stream>>>(m, V, Wv, d_vnrm,
+                                                    basis + static_cast<size_t>(j + 1) * n);
+      FH_HIP_TRY(hipGetLastError());
+      FH_HIP_TRY(hipMemcpyAsync(hh.data(), d_h, hh.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+      FH_HIP_TRY(hipMemcpyAsync(vn.data(), d_vnrm, V * sizeof(double), hipMemcpyDeviceToHost, stream));
+      FH_HIP_TRY(hipStreamSynchronize(stream));
+      for (int v = 0; v < V; ++v) {
+        if (done[v]) continue;
+        cd *Hv = H.data() + static_cast<size_t>(v) * (R1 + 1) * R1;  // column-major, ld R1 + 1
+        cd *gv = g.data() + static_cast<size_t>(v) * (R1 + 1);
+        double *cv = cs.data() + static_cast<size_t>(v) * R1;
+        cd *sv = sn.data() + static_cast<size_t>(v) * R1;
+        for (int i = 0; i <= j; ++i) {
+          const size_t a = 2 * (static_cast<size_t>(v) * hld + i);
+          const size_t b = a + 2 * static_cast<size_t>(V) * hld;
+          Hv[static_cast<size_t>(j) * (R1 + 1) + i] = cd(hh[a] + hh[b], hh[a + 1] + hh[b + 1]);
+        }
+        const double hn = std::sqrt(vn[v]);
+        Hv[static_cast<size_t>(j) * (R1 + 1) + j + 1] = cd(hn, 0.0);
+        for (int i = 0; i < j; ++i) {  // earlier rotations
+          cd &x0 = Hv[static_cast<size_t>(j) * (R1 + 1) + i];
+          cd &x1 = Hv[static_cast<size_t>(j) * (R1 + 1) + i + 1];
+          const cd t = cv[i] * x0 + sv[i] * x1;
+          x1 = -std::conj(sv[i]) * x0 + cv[i] * x1;
+          x0 = t;
+        }
+        cd &a0 = Hv[static_cast<size_t>(j) * (R1 + 1) + j];
+        cd &b0 = Hv[static_cast<size_t>(j) * (R1 + 1) + j + 1];
+        const double na = std::abs(a0), r = std::hypot(na, std::abs(b0));
+        if (r == 0.0 || !std::isfinite(r)) {
+          done[v] = 1;
+          continue;
+        }
+        const cd ph = na == 0.0 ? cd(1.0, 0.0) : a0 / na;
+        cv[j] = na / r;
+        sv[j] = ph * std::conj(b0) / r;
+        a0 = ph * r;
+        b0 = 0.0;
+        gv[j + 1] = -std::conj(sv[j]) * gv[j];
+        gv[j] = cv[j] * gv[j];
+        used[v] = j + 1;
+        if (std::abs(gv[j + 1]) <= kGmresTol * beta[v] || hn == 0.0) done[v] = 1;
+      }
+    }
+    steps_out = j;
+    // y_v = H_v^-1 g_v (upper triangular), then DX_v = sum_i y_i b_i
+    std::vector<double> y(2 * static_cast<size_t>(V) * hld, 0.0);
+    for (int v = 0; v < V; ++v) {
+      const int k = used[v];
+      const cd *Hv = H.data() + static_cast<size_t>(v) * (R1 + 1) * R1;
+      const cd *gv = g.data() + static_cast<size_t>(v) * (R1 + 1);
+      std::vector<cd> yy(k);
+      for (int i = k - 1; i >= 0; --i) {
+        cd acc = gv[i];
+        for (int c = i + 1; c < k; ++c) acc -= Hv[static_cast<size_t>(c) * (R1 + 1) + i] * yy[c];
+        yy[i] = acc / Hv[static_cast<size_t>(i) * (R1 + 1) + i];
+      }
+      for (int i = 0; i < k; ++i) {
+        y[2 * (static_cast<size_t>(v) * hld + i)] = yy[i].real();
+        y[2 * (static_cast<size_t>(v) * hld + i) + 1] = yy[i].imag();
+      }
+    }
+    FH_HIP_TRY(hipMemcpyAsync(d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    basis_combine_kernel<<<res_blocks, 256, 0, stream>>>(m, V, std::max(j, 1), basis, static_cast<int64_t>(n),
+                                                         d_y, hld, DX, +1);
+    FH_HIP_TRY(hipGetLastError());
+    FH_HIP_TRY(hipStreamSynchronize(stream));  // y is read before it goes out of scope
+    return FH_OK;
+  };
+
   // Acceptance.  With e = x* - x the error of x and r = b - Zee x = Zee e,
   // the corrected Z = Zpp - b^T x - x^T r misses Z* by exactly -r^T e, so
   // |dZ|_F <= |E|_F |R|_F.  While the iteration contracts by
@@ -1521,37 +1792,28 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   // the 1e-10 target), or when the first-order term |x^T r| / |Z| is below
   // kTol.
   const double kTol = 1e-13, kZTol = 1e-12;
-  // Split-bf16 outer updates first; the FP32 update if that does not converge
-  // (or if the split copies of one frequency exceed the 2 GiB range of a
-  // buffer descriptor, far above the dense path's 10 000 filaments).
-  const bool split_ok = split_blocks(L) * 4096 < (int64_t{1} << 31);
-  for (int attempt = split_ok ? 0 : 1; attempt < 2; ++attempt) {
-    const bool split = attempt == 0;
-    out.factor_arith = split ? 2 : 1;
-    FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
-    if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof,
-                               split ? &sb : nullptr)))
-      return st;
-    FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
-    FH_HIP_TRY(hipMemsetAsync(P, 0, n * sizeof(double), stream));
-    // r = b - Zee 0 = b
-    residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
-                                                    in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
-    FH_HIP_TRY(hipGetLastError());
-    out.singular.assign(K, 0);
-    out.refine_steps.assign(K, 0);
-    out.refine_change.assign(K, INFINITY);
-    out.converged.assign(K, 0);
-    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0), first_dx(K, 0.0);
-    for (int it = 1; it <= kMaxRefine; ++it) {
-      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
-      if ((st = factor_solve<float>(L, F, V, DX, P, stream))) return st;
+  out.singular.assign(K, 0);
+  out.refine_steps.assign(K, 0);
+  out.refine_change.assign(K, INFINITY);
+  out.converged.assign(K, 0);
+  out.stage.assign(K, -1);
+  std::vector<double> first_dx(K, 0.0);
+
+  // Refinement steps from the current X with the current factor, classical
+  // (dx = M^-1 r) or GMRES-IR; marks the frequencies that converge.
+  auto refine = [&](bool gmres, int stage, int max_steps) -> fh_status {
+    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0);
+    fh_status s2;
+    for (int it = 1; it <= max_steps; ++it) {
+      if (gmres) {
+        int inner = 0;
+        if ((s2 = gmres_correction(inner))) return s2;
+      } else {
+        axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
+        if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
+      }
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
-      lx_kernel<<<dim3(itiles, qtiles, lx_slices), 256, 0, stream>>>(in.d_lmesh, lds, m, Q, X,
-                                                                     lx_part, lx_slice_len);
-      lx_reduce_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), lx_slices, lx_part, P);
-      residual_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_ree_ptr, in.d_ree_col,
-                                                      in.d_ree_val, in.d_rep, in.d_lmesh, lds, X, P, R);
+      if ((s2 = residual())) return s2;
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
                                                         in.d_lmesh, lds, X, R, d_z, metric);
       norms_kernel<<<V, 256, 0, stream>>>(m, DX, R, d_nrm);
@@ -1570,7 +1832,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
         for (int q = 0; q < p * p; ++q)
           worst = std::max(worst, h_metric[static_cast<size_t>(f) * p * p + q]);
         if (!(worst == worst)) worst = INFINITY;
-        out.refine_steps[f] = it;
+        out.refine_steps[f] += 1;
         out.refine_change[f] = worst;
         double sdx = 0.0, sr = 0.0, sz = 0.0;
         for (int q = 0; q < p; ++q) {
@@ -1584,7 +1846,8 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
         const double ndx = std::sqrt(sdx), nr = std::sqrt(sr), nz = std::sqrt(sz);
         double bound = INFINITY, x_bound = INFINITY;
         bool stalled = false;
-        if (it == 1) first_dx[f] = ndx;  // X starts at 0: the first correction is X
+        // X starts at 0: the first correction of the first stage is X
+        if (first_dx[f] == 0.0 && std::isfinite(ndx)) first_dx[f] = ndx;
         if (it >= 2 && prev_dx[f] > 0.0) {
           const double rho = ndx / prev_dx[f];
           if (rho <= 0.5) x_bound = rho / (1.0 - rho) * ndx;
@@ -1595,33 +1858,76 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
         // Converged when the bound on the error of Z is small, when the
         // first-order term is negligible, or when that term has stopped
         // shrinking at a level the corrected Z absorbs.
-        bool done = bound <= kZTol || worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f]));
+        bool done = std::isfinite(nz) && h_sing[f] == 0 &&
+                    (bound <= kZTol || worst <= kTol || (worst <= 1e-9 && !(worst < 0.5 * prev[f])));
         // x itself to working accuracy when asked for (it enters gradients
         // to first order): the correction's bound under 1e-14 |x|, or the
         // contraction stalled at the rounding level, or x exact.
         if (done && in.tight_x) done = ndx == 0.0 || x_bound <= 1e-14 * first_dx[f] || stalled;
-        if (done)
+        if (done) {
           out.converged[f] = 1;
-        else
+          out.stage[f] = stage;
+        } else {
           all = false;
+        }
         prev[f] = worst;
       }
       if (all) break;
       // A contraction slower than 1/4 per step will not reach the tolerance
-      // within the budget: stop early and let the caller use more precision.
-      if (split && it >= 3) {
+      // within the budget: stop early and move on to the next stage.
+      if (it >= 3) {
         bool slow = false;
         for (int f = 0; f < K; ++f)
           if (!out.converged[f] && !(out.refine_change[f] < 1e-6)) slow = true;
         if (slow) break;
       }
     }
-    bool ok = true;
-    for (int f = 0; f < K; ++f) ok = ok && out.converged[f];
-    if (ok) break;
+    return FH_OK;
+  };
+
+  // The stages, in order: split-bf16 factor with classical refinement, the
+  // same factor under GMRES-IR, the FP32 factor with classical refinement,
+  // the FP32 factor under GMRES-IR (in.refine_stages selects which run).
+  // The split copies of one frequency must fit the 2 GiB range of a buffer
+  // descriptor (far above the dense path's 10 000 filaments).
+  const bool split_ok = split_blocks(L) * 4096 < (int64_t{1} << 31);
+  int factored = -1;  // 0 split, 1 FP32
+  bool started = false;
+  for (int stage = 0; stage < 4; ++stage) {
+    if (!(in.refine_stages & (1u << stage))) continue;
+    const int want = stage < 2 ? 0 : 1;
+    if (want == 0 && !split_ok) continue;
+    bool all = true;
+    for (int f = 0; f < K; ++f) all = all && out.converged[f];
+    if (all) break;
+    if (factored != want) {
+      FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
+      if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof, want == 0 ? &sb : nullptr)))
+        return st;
+      FH_HIP_TRY(hipMemcpyAsync(h_sing.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost, stream));
+      factored = want;
+      out.factor_arith = want == 0 ? 2 : 1;
+      if (!started) {
+        FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
+        started = true;
+      } else {
+        // restart the frequencies not yet converged from x = 0: an earlier
+        // factor may have left them non-finite
+        for (int f = 0; f < K; ++f) h_reset[f] = out.converged[f] ? 0 : 1;
+        FH_HIP_TRY(hipMemcpyAsync(d_reset, h_reset.data(), K * sizeof(int), hipMemcpyHostToDevice, stream));
+        reset_vectors_kernel<<<vec_blocks, 256, 0, stream>>>(m, p, V, d_reset, X);
+        FH_HIP_TRY(hipGetLastError());
+        for (int f = 0; f < K; ++f)
+          if (!out.converged[f]) first_dx[f] = 0.0;
+      }
+      if ((st = residual())) return st;
+      FH_HIP_TRY(hipStreamSynchronize(stream));  // h_sing, h_reset
+    }
+    const bool gmres = stage == 1 || stage == 3;
+    if ((st = refine(gmres, stage, gmres ? kMaxGmresRefine : kMaxRefine))) return st;
   }
-  FH_HIP_TRY(hipMemcpyAsync(out.singular.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost,
-                            stream));
+  // A zero pivot of an FP32 factor is not a singular system: it is reported
+  // as not converged, for the caller to solve in FP64.
   if (in.d_x_out)
     FH_HIP_TRY(hipMemcpyAsync(in.d_x_out, X, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   FH_HIP_TRY(hipStreamSynchronize(stream));

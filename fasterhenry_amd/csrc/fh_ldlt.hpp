@@ -30,6 +30,10 @@ struct DenseSolveInputs {
   // refined form: also refine x to working accuracy before stopping (the
   // default stops when Z, which is stationary in x, has converged)
   bool tight_x = false;
+  // refined form: the stages allowed, bit k = stage k (0 split-bf16 factor
+  // with classical refinement, 1 the same factor under GMRES-IR, 2 the FP32
+  // factor with classical refinement, 3 the FP32 factor under GMRES-IR)
+  unsigned refine_stages = 15u;
 };
 
 // Grow-only device workspace (through device_alloc, so that a batch
@@ -104,6 +108,7 @@ struct BatchResult {
   std::vector<int> refine_steps;     // per frequency (refined form)
   std::vector<double> refine_change; // per frequency: final |x^T r| / |b^T x|
   std::vector<int> converged;        // per frequency (refined form)
+  std::vector<int> stage;            // per frequency: refinement stage that converged, -1 none
   int factor_arith = 0;              // 0 FP64, 1 FP32 MFMA, 2 FP32 with split-bf16 updates
 };
 

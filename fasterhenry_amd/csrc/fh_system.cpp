@@ -152,6 +152,7 @@ struct fh_system {
   // updates
   static constexpr int kParts = kSweepPartsMax;
   int sweep_parts = 2;  // two measured fastest on MI355X (four contend)
+  uint32_t refine_stages = FH_REFINE_ALL;
   hipStream_t part_stream[kParts] = {nullptr, nullptr, nullptr, nullptr};
   LdltWorkspace part_ws[kParts];
   double inductance_s = 0.0, assembly_s = 0.0;
@@ -263,6 +264,45 @@ fh_status upload_mask(const fh_pair_mask *mask, DeviceArray<int32_t> &d_group, D
   return FH_OK;
 }
 
+// The frequencies of a refined chunk that no FP32 stage accepted, solved by
+// the plain FP64 form (ldlt_solve_batch) in groups that fit the memory
+// budget; their Z goes to z_chunk (kc x p x p {re, im}, host) and res marks
+// them converged at stage 4 (singular flags from the FP64 pivots).
+fh_status fp64_resolve(fh_system *s, const DenseSolveInputs &in, const double *omegas, size_t kc,
+                       BatchResult &res, double *z_chunk) {
+  std::vector<size_t> left;
+  for (size_t q = 0; q < kc; ++q)
+    if (!res.converged[q]) left.push_back(q);
+  if (left.empty()) return FH_OK;
+  const size_t p = in.p;
+  const size_t per = ldlt_bytes_per_frequency(in.m, p, false);
+  const size_t group = std::min(left.size(), std::max<size_t>(1, sweep_budget_bytes() / per));
+  DeviceArray<double> d_w, d_z;
+  fh_status st;
+  if ((st = d_w.allocate(group)) || (st = d_z.allocate(group * p * p * 2))) return st;
+  LdltWorkspace ws;
+  std::vector<double> w(group), z(group * p * p * 2);
+  for (size_t g0 = 0; g0 < left.size(); g0 += group) {
+    const size_t gn = std::min(group, left.size() - g0);
+    for (size_t k = 0; k < gn; ++k) w[k] = omegas[left[g0 + k]];
+    FH_HIP_TRY(hipMemcpyAsync(d_w.ptr, w.data(), gn * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    BatchResult r64;
+    if ((st = ldlt_solve_batch(in, d_w.ptr, static_cast<int>(gn), d_z.ptr, r64, ws, s->stream, nullptr)))
+      return st;
+    FH_HIP_TRY(hipMemcpyAsync(z.data(), d_z.ptr, gn * p * p * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              s->stream));
+    FH_HIP_TRY(hipStreamSynchronize(s->stream));
+    for (size_t k = 0; k < gn; ++k) {
+      const size_t q = left[g0 + k];
+      std::memcpy(z_chunk + 2 * p * p * q, z.data() + 2 * p * p * k, 2 * p * p * sizeof(double));
+      res.singular[q] = r64.singular[k];
+      res.converged[q] = 1;
+      res.stage[q] = 4;
+    }
+  }
+  return FH_OK;
+}
+
 // x = Zee^-1 Zep (2 K p vectors of m: real then imaginary part of column a
 // at frequency k, vector 2 (k p + a)) by the refined solve, refined until
 // the correction is at the rounding level (tight_x); first failure in list
@@ -293,6 +333,7 @@ fh_status refined_x(fh_system *s, const double *freqs, size_t K, std::vector<dou
   in.d_rpp = s->d_rpp.ptr;
   in.d_x_out = d_x.ptr;
   in.tight_x = true;
+  in.refine_stages = s->refine_stages & 15u;
   std::vector<double> omegas(chunk), z(2 * chunk * p * p);
   for (size_t c0 = 0; c0 < K; c0 += chunk) {
     const size_t kc = std::min(chunk, K - c0);
@@ -711,6 +752,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       info->gmres_residual_max = res_max;
       info->operator_applies = static_cast<int64_t>(it.applies());
       info->operator_apply_ms = s->profiling ? it.apply_ms() : 0.0;
+      info->refine_stage = -1;
     }
     return FH_OK;
   }
@@ -721,6 +763,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
   int refine_steps_max = 0;
   double refine_change_max = 0.0;
   int factor_arith = s->precision == FH_FP32_REFINED ? 1 : 0;
+  int refine_stage = -1;
   if (m == 0) {
     // No internal loops: Z = Zpp, nothing to solve (solve.rs:1159-1161).
     std::vector<double> lm(p * p);
@@ -758,6 +801,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     in.d_ree_val = s->d_ree_val.ptr;
     in.d_rep = s->d_rep.ptr;
     in.d_rpp = s->d_rpp.ptr;
+    in.refine_stages = s->refine_stages & 15u;
     std::vector<double> omegas(chunk);
     for (size_t c0 = 0; c0 < K; c0 += chunk) {
       const size_t kc = std::min(chunk, K - c0);
@@ -825,6 +869,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
           res.refine_steps.insert(res.refine_steps.end(), rs[q].refine_steps.begin(), rs[q].refine_steps.end());
           res.refine_change.insert(res.refine_change.end(), rs[q].refine_change.begin(), rs[q].refine_change.end());
           res.converged.insert(res.converged.end(), rs[q].converged.begin(), rs[q].converged.end());
+          res.stage.insert(res.stage.end(), rs[q].stage.begin(), rs[q].stage.end());
           res.factor_arith = std::max(res.factor_arith, rs[q].factor_arith);
           if (pp) {
             extra_ms += ms[q];
@@ -843,7 +888,14 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
       FH_HIP_TRY(hipMemcpyAsync(z_out + 2 * p * p * c0, d_z.ptr, kc * p * p * 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s->stream));
       FH_HIP_TRY(hipStreamSynchronize(s->stream));
+      // the last stage: frequencies no FP32 stage solved, by the plain FP64
+      // L D L^T (bordered elimination, no refinement)
+      if (refined && (s->refine_stages & FH_REFINE_FP64)) {
+        if ((st = fp64_resolve(s, in, omegas.data(), kc, res, z_out + 2 * p * p * c0))) return st;
+      }
       factor_arith = std::max(factor_arith, res.factor_arith);
+      if (refined)
+        for (size_t q = 0; q < kc; ++q) refine_stage = std::max(refine_stage, res.stage[q]);
       for (size_t q = 0; q < kc; ++q) {
         refine_steps_max = std::max(refine_steps_max, res.refine_steps[q]);
         refine_change_max = std::max(refine_change_max, res.refine_change[q]);
@@ -878,6 +930,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->refine_steps_max = refine_steps_max;
     info->factor_arith = factor_arith;
     info->refine_change_max = refine_change_max;
+    info->refine_stage = refine_stage;
     info->update_kernel_ms = pp ? prof.total_ms() + extra_ms : 0.0;
     info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used + extra_launches) : 0;
     prof.flops += extra_flops;
@@ -891,6 +944,15 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->operator_applies = 0;
     info->operator_apply_ms = 0.0;
   }
+  return FH_OK;
+}
+
+fh_status fh_system_set_refinement(fh_system *s, uint32_t stages) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  if (stages == 0 || (stages & ~FH_REFINE_ALL))
+    return set_error(FH_E_INVALID_ARGUMENT, "stages must be a non-empty mask of FH_REFINE_* bits");
+  s->refine_stages = stages;
   return FH_OK;
 }
 

@@ -21,6 +21,12 @@ FH_OK = 0
 FH_FP64_PLAIN = 0
 FH_FP32_REFINED = 1
 FH_DENSE_PATH_MAX_FILAMENTS = 10000
+FH_REFINE_SPLIT_CLASSICAL = 1
+FH_REFINE_SPLIT_GMRES = 2
+FH_REFINE_FP32_CLASSICAL = 4
+FH_REFINE_FP32_GMRES = 8
+FH_REFINE_FP64 = 16
+FH_REFINE_ALL = 31
 
 SYMBOLS = [
     "fh_resolve", "fh_system_create", "fh_sweep", "fh_impedance", "fh_system_set_precision",
@@ -34,7 +40,7 @@ SYMBOLS = [
     "fh_zc_mat", "fh_spice_subckt", "fh_subckt_name_from_path", "fh_system_create_coupled",
     "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
     "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
-    "fh_system_set_sweep_parts", "fh_partial_inductance_noalloc",
+    "fh_system_set_sweep_parts", "fh_partial_inductance_noalloc", "fh_system_set_refinement",
 ]
 
 _ERRORS = {
@@ -62,7 +68,8 @@ class SweepInfo(C.Structure):
                 ("update_kernel_flops", C.c_double), ("update_kernel_unit_flops", C.c_double),
                 ("solver", C.c_int32), ("gmres_iterations_max", C.c_int32),
                 ("gmres_iterations", C.c_int64), ("gmres_residual_max", C.c_double),
-                ("operator_applies", C.c_int64), ("operator_apply_ms", C.c_double)]
+                ("operator_applies", C.c_int64), ("operator_apply_ms", C.c_double),
+                ("refine_stage", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -168,6 +175,8 @@ def lib() -> C.CDLL:
                                C.POINTER(SweepInfo)]
         L.fh_partial_inductance.restype = st
         L.fh_partial_inductance.argtypes = [vp, sz, vp, vp]
+        L.fh_system_set_refinement.restype = st
+        L.fh_system_set_refinement.argtypes = [vp, C.c_uint32]
         L.fh_partial_inductance_noalloc.restype = st
         L.fh_partial_inductance_noalloc.argtypes = [vp, sz, vp, vp, vp]
         L.fh_discretize.restype = st

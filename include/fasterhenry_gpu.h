@@ -186,6 +186,9 @@ typedef struct fh_sweep_info {
   double gmres_residual_max;
   int64_t operator_applies;
   double operator_apply_ms;
+  /* refined form: the last refinement stage any frequency needed (0-4, see
+   * fh_system_set_refinement); -1 for the plain form and the iterative path */
+  int32_t refine_stage;
 } fh_sweep_info;
 
 typedef struct fh_system fh_system;
@@ -246,6 +249,26 @@ fh_status fh_system_filaments(const fh_system *system, fh_filament *out);
  * depend on it beyond the refinement's rounding (each frequency is refined
  * to the same acceptance bound). */
 fh_status fh_system_set_sweep_parts(fh_system *system, int32_t parts);
+
+/* Refinement stages of the FP32_REFINED form (no counterpart in the
+ * reference, whose LU fails only on an exactly zero pivot, dense.rs:43-84):
+ * each frequency goes through the allowed stages in this order until its Z
+ * is accepted; 0 split-bf16 factorization with classical refinement, 1 the
+ * same factorization under GMRES-IR (GMRES on the preconditioned correction
+ * equation, FP64 residuals), 2 the FP32 factorization with classical
+ * refinement, 3 the FP32 factorization under GMRES-IR, 4 a plain FP64
+ * L D L^T solve of the frequencies still left (sweeps only; fh_grad and
+ * fh_currents need x and stop at stage 3).  A frequency no allowed stage
+ * solves is FH_E_NOT_CONVERGED.  Default FH_REFINE_ALL; other masks are for
+ * tests and studies.  fh_sweep_info.refine_stage reports the last stage any
+ * frequency needed. */
+#define FH_REFINE_SPLIT_CLASSICAL 1u
+#define FH_REFINE_SPLIT_GMRES 2u
+#define FH_REFINE_FP32_CLASSICAL 4u
+#define FH_REFINE_FP32_GMRES 8u
+#define FH_REFINE_FP64 16u
+#define FH_REFINE_ALL 31u
+fh_status fh_system_set_refinement(fh_system *system, uint32_t stages);
 
 /* Dense systems only (FH_E_UNSUPPORTED for an iterative one). */
 fh_status fh_system_inductance(const fh_system *system, double *L_out);
