@@ -25,7 +25,6 @@
 // filament order), FFT convolution (rocFFT, FP64), gather (one thread per
 // filament).  Deterministic: no atomics on values.
 #include <hip/hip_runtime.h>
-#include <rocfft/rocfft.h>
 
 #include <algorithm>
 #include <atomic>
@@ -584,24 +583,39 @@ struct SourceSet {
 __device__ inline float2 to_c(double a, double b, float2) { return make_float2(static_cast<float>(a), static_cast<float>(b)); }
 __device__ inline double2 to_c(double a, double b, double2) { return make_double2(a, b); }
 
+// The batch's per-filament source factors packed together: pk[f][s] =
+// (l_c(f), x_v(f)) for source s = (v, c), so that the spread reads one
+// contiguous record per projection entry instead of gathering the direction
+// and every vector separately.
+__global__ void pack_sources_kernel(const double *__restrict__ dirs, const double *__restrict__ x,
+                                    int64_t n, SourceSet ss, double2 *__restrict__ pk) {
+  const int64_t f = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (f >= n) return;
+#pragma unroll
+  for (int s = 0; s < kBatchSources; ++s)
+    if (s < ss.count) pk[f * ss.count + s] = make_double2(dirs[3 * f + ss.comp[s]], x[ss.vec[s] * n + f]);
+}
+
 // Sources on the compact grid: q_s(g) = sum over the transposed projection
 // (filaments in order) of (w l_c) x_v, in FP64, stored as channel values.
 template <typename T2>
 __global__ void __launch_bounds__(256) spread_compact_kernel(
     T2 *__restrict__ grid, const int64_t *__restrict__ soff, const uint32_t *__restrict__ sfil,
-    const double *__restrict__ sw, const double *__restrict__ dirs, const double *__restrict__ x,
-    int64_t n, int64_t G, SourceSet ss) {
+    const double *__restrict__ sw, const double2 *__restrict__ pk, int64_t G, SourceSet ss) {
   const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (g >= G) return;
   double acc[kBatchSources];
 #pragma unroll
   for (int s = 0; s < kBatchSources; ++s) acc[s] = 0.0;
   for (int64_t e = soff[g]; e < soff[g + 1]; ++e) {
-    const int64_t f = sfil[e];
+    const double2 *q = pk + static_cast<int64_t>(sfil[e]) * ss.count;
     const double w = sw[e];
 #pragma unroll
     for (int s = 0; s < kBatchSources; ++s)
-      if (s < ss.count) acc[s] += w * dirs[3 * f + ss.comp[s]] * x[ss.vec[s] * n + f];
+      if (s < ss.count) {
+        const double2 dx = q[s];
+        acc[s] += w * dx.x * dx.y;
+      }
   }
 #pragma unroll
   for (int c = 0; c < kBatchSources / 2; ++c)
@@ -672,11 +686,154 @@ void parallel_for(size_t n, F f) {
   for (auto &th : pool) th.join();
 }
 
-fh_status rocfft_check(rocfft_status s, const char *what) {
-  if (s != rocfft_status_success)
-    return set_error(FH_E_HIP, std::string("rocFFT ") + what + " failed (" + std::to_string(s) + ")");
+// ---------------------------------------------------------------------------
+// The FP64 3-D FFT of the padded grid for grids with an axis too long for
+// one LDS line (more than 8 191 points): Stockham autosort passes through
+// global memory, one launch per radix (4, 2, 3, 5) and axis, ping-ponging
+// between the grid and a work buffer.  Slower than the LDS line passes, and
+// only used for such extreme aspect ratios.
+
+// v[0..R) <- DFT_R with kernel exp(sgn 2 pi i n k / R).
+template <int R>
+__device__ inline void small_dft(double2 *v, double sgn) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = make_double2(a.x + b.x, a.y + b.y);
+    v[1] = make_double2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const double2 t0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const double2 t1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const double2 t2 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const double2 d = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    const double2 wd = make_double2(-sgn * d.y, sgn * d.x);  // (sgn i) d
+    v[0] = make_double2(t0.x + t2.x, t0.y + t2.y);
+    v[2] = make_double2(t0.x - t2.x, t0.y - t2.y);
+    v[1] = make_double2(t1.x + wd.x, t1.y + wd.y);
+    v[3] = make_double2(t1.x - wd.x, t1.y - wd.y);
+  } else if constexpr (R == 3) {
+    const double h = 0.86602540378443864676;  // sqrt(3) / 2
+    const double2 t = make_double2(v[1].x + v[2].x, v[1].y + v[2].y);
+    const double2 u = make_double2(v[1].x - v[2].x, v[1].y - v[2].y);
+    const double2 m = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+    const double2 iu = make_double2(-sgn * h * u.y, sgn * h * u.x);  // sgn i (sqrt3/2) u
+    v[0] = make_double2(v[0].x + t.x, v[0].y + t.y);
+    v[1] = make_double2(m.x + iu.x, m.y + iu.y);
+    v[2] = make_double2(m.x - iu.x, m.y - iu.y);
+  } else {  // R == 5
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;  // cos 2pi/5, cos 4pi/5
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;   // sin 2pi/5, sin 4pi/5
+    const double2 a1 = make_double2(v[1].x + v[4].x, v[1].y + v[4].y);
+    const double2 b1 = make_double2(v[1].x - v[4].x, v[1].y - v[4].y);
+    const double2 a2 = make_double2(v[2].x + v[3].x, v[2].y + v[3].y);
+    const double2 b2 = make_double2(v[2].x - v[3].x, v[2].y - v[3].y);
+    const double2 x0 = v[0];
+    const double2 p1 = make_double2(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+    const double2 p2 = make_double2(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+    // sgn i (s1 b1 + s2 b2) and sgn i (s2 b1 - s1 b2)
+    const double2 q1 = make_double2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+    const double2 q2 = make_double2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+    const double2 iq1 = make_double2(-sgn * q1.y, sgn * q1.x), iq2 = make_double2(-sgn * q2.y, sgn * q2.x);
+    v[0] = make_double2(x0.x + a1.x + a2.x, x0.y + a1.y + a2.y);
+    v[1] = make_double2(p1.x + iq1.x, p1.y + iq1.y);
+    v[4] = make_double2(p1.x - iq1.x, p1.y - iq1.y);
+    v[2] = make_double2(p2.x + iq2.x, p2.y + iq2.y);
+    v[3] = make_double2(p2.x - iq2.x, p2.y - iq2.y);
+  }
+}
+
+// One radix-R Stockham pass along an axis of length M and element stride s
+// (lines = total / M): y[d + r Ns] = DFT_R(x[j + r M / R] w^(k r)), k = j mod Ns,
+// d = (j / Ns) Ns R + k.  Consecutive threads take consecutive inner offsets,
+// so both the strided axes and the contiguous one load coalesced.
+template <int R>
+__global__ void stockham_pass_kernel(const double2 *__restrict__ x, double2 *__restrict__ y, int64_t total,
+                                     int64_t M, int64_t s, int64_t Ns, const double2 *__restrict__ tw,
+                                     double sgn) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int64_t mr = M / R;
+  if (t >= total / R) return;
+  const int64_t inner = t % s, rest = t / s;
+  const int64_t j = rest % mr, outer = rest / mr;
+  const int64_t base = outer * M * s + inner;
+  const int64_t k = j % Ns;
+  const int64_t step = M / (Ns * R);
+  double2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    double2 a = x[base + (j + r * mr) * s];
+    if (r > 0 && k > 0) {
+      const double2 w = tw[k * r * step];
+      const double wy = sgn < 0.0 ? w.y : -w.y;  // the table holds exp(-2 pi i q / M)
+      a = make_double2(a.x * w.x - a.y * wy, a.x * wy + a.y * w.x);
+    }
+    v[r] = a;
+  }
+  small_dft<R>(v, sgn);
+  const int64_t d = (j / Ns) * Ns * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) y[base + (d + r * Ns) * s] = v[r];
+}
+
+struct StockhamAxis {
+  int64_t M = 0;
+  std::vector<int> radix;
+  DeviceArray<double2> tw;  // exp(-2 pi i q / M), q < M
+};
+
+fh_status stockham_axis_plan(int64_t M, StockhamAxis &ax, hipStream_t stream) {
+  ax.M = M;
+  ax.radix.clear();
+  int64_t r = M;
+  for (const int f : {4, 2, 3, 5})
+    while (r % f == 0) {
+      ax.radix.push_back(f);
+      r /= f;
+    }
+  if (r != 1) return set_error(FH_E_PFFT, "padded FFT length " + std::to_string(M) + " is not 2-3-5 smooth");
+  std::vector<double2> h(static_cast<size_t>(M));
+  for (int64_t q = 0; q < M; ++q) {
+    const double a = -6.283185307179586476925 * static_cast<double>(q) / static_cast<double>(M);
+    h[q] = make_double2(std::cos(a), std::sin(a));
+  }
+  fh_status st;
+  if ((st = ax.tw.allocate(static_cast<size_t>(M)))) return st;
+  FH_HIP_TRY(hipMemcpyAsync(ax.tw.ptr, h.data(), M * sizeof(double2), hipMemcpyHostToDevice, stream));
+  FH_HIP_TRY(hipStreamSynchronize(stream));  // h is read before it goes out of scope
   return FH_OK;
 }
+
+// The unnormalized 3-D DFT (kernel exp(-+2 pi i k.n / M)) of data
+// [M0][M1][M2], in place (work: the same size).
+fh_status stockham_3d(double2 *data, double2 *work, const int64_t dims[3], StockhamAxis ax[3],
+                      bool forward, hipStream_t stream) {
+  const int64_t total = dims[0] * dims[1] * dims[2];
+  const double sgn = forward ? -1.0 : 1.0;
+  double2 *src = data, *dst = work;
+  for (int a = 2; a >= 0; --a) {
+    const int64_t M = dims[a];
+    const int64_t s = a == 2 ? 1 : (a == 1 ? dims[2] : dims[1] * dims[2]);
+    int64_t Ns = 1;
+    for (const int R : ax[a].radix) {
+      const int64_t threads = total / R;
+      const unsigned blocks = static_cast<unsigned>((threads + 255) / 256);
+      if (R == 4)
+        stockham_pass_kernel<4><<<blocks, 256, 0, stream>>>(src, dst, total, M, s, Ns, ax[a].tw.ptr, sgn);
+      else if (R == 2)
+        stockham_pass_kernel<2><<<blocks, 256, 0, stream>>>(src, dst, total, M, s, Ns, ax[a].tw.ptr, sgn);
+      else if (R == 3)
+        stockham_pass_kernel<3><<<blocks, 256, 0, stream>>>(src, dst, total, M, s, Ns, ax[a].tw.ptr, sgn);
+      else
+        stockham_pass_kernel<5><<<blocks, 256, 0, stream>>>(src, dst, total, M, s, Ns, ax[a].tw.ptr, sgn);
+      FH_HIP_TRY(hipGetLastError());
+      Ns *= R;
+      std::swap(src, dst);
+    }
+  }
+  if (src != data)
+    FH_HIP_TRY(hipMemcpyAsync(data, src, total * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  return FH_OK;
+}
+
 
 }  // namespace
 
@@ -694,31 +851,21 @@ struct PfftImpl {
   DeviceArray<double> spectrum;
   DeviceArray<double2> buffer;
   DeviceArray<double> far;
-  DeviceArray<char> work;
-  rocfft_plan fwd = nullptr, inv = nullptr;
-  rocfft_execution_info info = nullptr;
+  DeviceArray<double2> packed;  // n x kBatchSources source factors
+  DeviceArray<double2> work;
+  StockhamAxis axes[3];
   size_t far_cap = 0;
   PrunedConvolver conv;
   bool use_conv = false;
   int fft_kind = 0;
 
-  ~PfftImpl() {
-    if (fwd) rocfft_plan_destroy(fwd);
-    if (inv) rocfft_plan_destroy(inv);
-    if (info) rocfft_execution_info_destroy(info);
-  }
-
   int64_t padded_total() const { return padded[0] * padded[1] * padded[2]; }
   int64_t grid_total() const { return g.dims[0] * g.dims[1] * g.dims[2]; }
 
   fh_status fft(bool forward, hipStream_t stream) {
-    FH_HIP_TRY(hipGetLastError());
-    fh_status st;
-    if ((st = rocfft_check(rocfft_execution_info_set_stream(info, stream), "set_stream")))
-      return st;
-    void *bufs[1] = {buffer.ptr};
-    return rocfft_check(rocfft_execute(forward ? fwd : inv, bufs, nullptr, info), "execute");
+    return stockham_3d(buffer.ptr, work.ptr, padded, axes, forward, stream);
   }
+
 };
 
 PfftOperator::PfftOperator() : impl_(new PfftImpl) {}
@@ -1243,37 +1390,15 @@ fh_status PfftOperator::build(const std::vector<fh_filament> &F, const fh_pfft_p
       return FH_OK;
     }
   }
-  // a padded axis too long for one LDS line: library 3-D FFT on the padded grid
+  // a padded axis too long for one LDS line: the whole padded grid through
+  // global-memory Stockham passes
   op.fft_kind = 2;
   const int64_t M = op.padded_total();
-  if ((st = op.buffer.allocate(static_cast<size_t>(M))) || (st = op.spectrum.allocate(static_cast<size_t>(M))))
+  if ((st = op.buffer.allocate(static_cast<size_t>(M))) || (st = op.spectrum.allocate(static_cast<size_t>(M))) ||
+      (st = op.work.allocate(static_cast<size_t>(M))))
     return st;
-  static std::once_flag rocfft_once;
-  static rocfft_status rocfft_setup_status = rocfft_status_success;
-  std::call_once(rocfft_once, [] { rocfft_setup_status = rocfft_setup(); });
-  if ((st = rocfft_check(rocfft_setup_status, "setup"))) return st;
-  const size_t lengths[3] = {static_cast<size_t>(op.padded[2]), static_cast<size_t>(op.padded[1]),
-                             static_cast<size_t>(op.padded[0])};
-  if ((st = rocfft_check(rocfft_plan_create(&op.fwd, rocfft_placement_inplace,
-                                            rocfft_transform_type_complex_forward,
-                                            rocfft_precision_double, 3, lengths, 1, nullptr),
-                         "plan_create")))
-    return st;
-  if ((st = rocfft_check(rocfft_plan_create(&op.inv, rocfft_placement_inplace,
-                                            rocfft_transform_type_complex_inverse,
-                                            rocfft_precision_double, 3, lengths, 1, nullptr),
-                         "plan_create")))
-    return st;
-  size_t w1 = 0, w2 = 0;
-  rocfft_plan_get_work_buffer_size(op.fwd, &w1);
-  rocfft_plan_get_work_buffer_size(op.inv, &w2);
-  if ((st = rocfft_check(rocfft_execution_info_create(&op.info), "execution_info_create"))) return st;
-  if (std::max(w1, w2) > 0) {
-    if ((st = op.work.allocate(std::max(w1, w2)))) return st;
-    if ((st = rocfft_check(rocfft_execution_info_set_work_buffer(op.info, op.work.ptr, std::max(w1, w2)),
-                           "set_work_buffer")))
-      return st;
-  }
+  for (int a = 0; a < 3; ++a)
+    if ((st = stockham_axis_plan(op.padded[a], op.axes[a], stream))) return st;
   kernel_grid_kernel<<<blocks_for(M), 256, 0, stream>>>(op.buffer.ptr, g.dims[0], g.dims[1], g.dims[2],
                                                         op.padded[0], op.padded[1], op.padded[2], h);
   if ((st = op.fft(true, stream))) return st;
@@ -1298,6 +1423,8 @@ fh_status PfftOperator::apply(const double *d_x, double *d_y, int V, hipStream_t
     if ((st = op.far.allocate(need))) return st;
     op.far_cap = need;
   }
+  if (op.use_conv && op.packed.count < op.n * kBatchSources)
+    if ((st = op.packed.allocate(op.n * kBatchSources))) return st;
   zero_kernel<<<blocks_for(static_cast<int64_t>(need)), 256, 0, stream>>>(op.far.ptr, static_cast<int64_t>(need));
   const int S = V * op.ncomp;  // sources (v, c), v outer
   const GridDesc &g = op.g;
@@ -1313,16 +1440,18 @@ fh_status PfftOperator::apply(const double *d_x, double *d_y, int V, hipStream_t
       const int C = (ss.count + 1) / 2;
       if (op.conv.single()) {
         float2 *grid = static_cast<float2 *>(op.conv.grid());
+        pack_sources_kernel<<<blocks_for(n), 256, 0, stream>>>(op.dirs.ptr, d_x, n, ss, op.packed.ptr);
         spread_compact_kernel<float2><<<blocks_for(G), 256, 0, stream>>>(
-            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.dirs.ptr, d_x, n, G, ss);
+            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.packed.ptr, G, ss);
         FH_HIP_TRY(hipGetLastError());
         if ((st = op.conv.convolve(C, stream))) return st;
         gather_compact_kernel<float2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(
             grid, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, op.dirs.ptr, n, G, ss, op.far.ptr);
       } else {
         double2 *grid = static_cast<double2 *>(op.conv.grid());
+        pack_sources_kernel<<<blocks_for(n), 256, 0, stream>>>(op.dirs.ptr, d_x, n, ss, op.packed.ptr);
         spread_compact_kernel<double2><<<blocks_for(G), 256, 0, stream>>>(
-            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.dirs.ptr, d_x, n, G, ss);
+            grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.packed.ptr, G, ss);
         FH_HIP_TRY(hipGetLastError());
         if ((st = op.conv.convolve(C, stream))) return st;
         gather_compact_kernel<double2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(

@@ -102,8 +102,8 @@ typedef struct fh_pfft_stats {
   uint64_t near_entries;
   uint64_t memory_bytes; /* device memory held by the operator */
   /* the convolution used: 0 = FP64 pruned line FFTs, 1 = FP32 pruned line
-   * FFTs, 2 = FP64 library 3-D FFT (a padded axis too long for one line in
-   * LDS) */
+   * FFTs, 2 = FP64 3-D FFT by global-memory Stockham passes (a padded axis
+   * too long for one line in LDS) */
   int32_t fft_kind;
   /* lines per workgroup the set-up autotuner chose for the axis-1 forward,
    * axis-0 and axis-1 inverse line passes (0 when not tuned) */
