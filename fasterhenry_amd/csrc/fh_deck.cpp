@@ -19,7 +19,7 @@
 #include <string>
 #include <vector>
 
-#include "fh_internal.hpp"
+#include "fh_host.hpp"
 
 struct fh_deck {
   std::string title;

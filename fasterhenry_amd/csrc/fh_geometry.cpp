@@ -9,8 +9,10 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <new>
+#include <stdexcept>
 
-#include "fh_internal.hpp"
+#include "fh_host.hpp"
 
 namespace fh {
 namespace {
@@ -189,15 +191,39 @@ fh_status validate_geometry(const fh_problem &p) {
   return FH_OK;
 }
 
-// solve.rs:947-975 with filament.rs:320-359 and 478-530.
+namespace {
+fh_status discretize_checked(const fh_problem &p, std::vector<fh_filament> &out);
+}
+
+// solve.rs:947-975 with filament.rs:320-359 and 478-530.  A discretization
+// too large for host memory is FH_E_NO_MEMORY, not an exception through the
+// C ABI.
 fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out) {
+  try {
+    return discretize_checked(p, out);
+  } catch (const std::bad_alloc &) {
+    out.clear();
+    return set_error(FH_E_NO_MEMORY, "the discretization does not fit in host memory");
+  } catch (const std::length_error &) {
+    out.clear();
+    return set_error(FH_E_NO_MEMORY, "the discretization does not fit in host memory");
+  }
+}
+
+namespace {
+fh_status discretize_checked(const fh_problem &p, std::vector<fh_filament> &out) {
   fh_status st = validate_geometry(p);
   if (st) return st;
-  size_t total = 0;
-  for (size_t s = 0; s < p.n_segments; ++s)
-    total += static_cast<size_t>(p.grids[s].nw) * p.grids[s].nh;
+  // nw nh of one segment above 2^31 is DiscretizeError::Overflow
+  // (filament.rs:493-495), reported in segment order below; the reservation
+  // only counts the segments that pass that test.
+  uint64_t total = 0;
+  for (size_t s = 0; s < p.n_segments; ++s) {
+    const uint64_t c = static_cast<uint64_t>(p.grids[s].nw) * p.grids[s].nh;
+    if (c <= (uint64_t{1} << 31)) total += c;
+  }
   out.clear();
-  out.reserve(total);
+  out.reserve(static_cast<size_t>(std::min<uint64_t>(total, uint64_t{1} << 31)));
   std::vector<double> wc, we, hc, he;
   for (size_t s = 0; s < p.n_segments; ++s) {
     const fh_segment &seg = p.segments[s];
@@ -214,6 +240,9 @@ fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out)
                    12);
     }
     if (g.nw == 0 || g.nh == 0) return bad(FH_E_DISCRETIZE, "zero subdivision", 10);
+    if (static_cast<uint64_t>(g.nw) * g.nh > (uint64_t{1} << 31))
+      return bad(FH_E_DISCRETIZE,
+                 std::to_string(g.nw) + " x " + std::to_string(g.nh) + " filaments overflow", 11);
     const V3 a = load(p.nodes + 3 * seg.a), b = load(p.nodes + 3 * seg.b);
     V3 l, w, h;
     if (int e = segment_frame(a, b, seg, l, w, h)) return bad(FH_E_DISCRETIZE, kSegmentReasons[e], e);
@@ -253,6 +282,7 @@ fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out)
   }
   return FH_OK;
 }
+}  // namespace
 
 // The derivative of discretize_problem along one direction of the geometry:
 // node coordinates move by dn (n_nodes x 3), segment widths by dw and

@@ -9,21 +9,10 @@
 #include <vector>
 
 #include "fasterhenry_gpu.h"
+#include "fh_host.hpp"
 
 namespace fh {
 
-// Error state of the calling thread, surfaced by fh_last_error /
-// fh_error_detail.
-struct ErrorState {
-  std::string message;
-  int64_t index0 = 0, index1 = 0;
-  double value = 0.0;
-  int32_t code = 0;
-};
-ErrorState &error_state();
-fh_status set_error(fh_status status, const std::string &message, int64_t index0 = 0,
-                    int64_t index1 = 0, double value = 0.0, int32_t code = 0);
-void clear_error();
 
 #define FH_HIP_TRY(expr)                                                                \
   do {                                                                                  \
@@ -69,17 +58,7 @@ struct DeviceArray {
   }
 };
 
-// Sparse loop basis (mesh.rs:170-255), CSR by loop.
-struct LoopBasis {
-  std::vector<int64_t> row_ptr;  // loops + 1
-  std::vector<int32_t> branch;   // nnz
-  std::vector<double> sign;      // nnz, +-1
-  size_t loops = 0, internal = 0;
-};
 
-// Host geometry (fh_geometry.cpp).
-fh_status validate_geometry(const fh_problem &p);
-fh_status discretize_problem(const fh_problem &p, std::vector<fh_filament> &out);
 fh_status build_loop_basis(const fh_problem &p, const std::vector<size_t> &per_segment,
                            LoopBasis &out);
 // geometry.rs:223-276 frame check of one segment between points a and b: 0,

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "fh_internal.hpp"
+#include "fh_host.hpp"
 
 namespace fh {
 namespace {
