@@ -68,6 +68,36 @@ def meander_problem(bars: int):
     return g, [p], Discretization.uniform(SUBDIVISION_NW, SUBDIVISION_NH)
 
 
+def trace_meander(runs: int = 12, run_length: float = 8e-3, pitch: float = 0.4e-3,
+                  width: float = 0.2e-3, thickness: float = 35e-6, segments_per_run: int = 12,
+                  nw: int = 5, nh: int = 3, ratio: float = 2.0):
+    """A copper trace ``width`` x ``thickness`` folded into ``runs`` parallel
+    runs of ``run_length`` at ``pitch``, laid out as meander() lays its
+    runs: each run cut into ``segments_per_run`` segments and followed by a
+    one-pitch jog (12 x 12 + 12 = 156 segments); every segment ``nw`` x ``nh``
+    graded toward the surfaces by ``ratio`` (2 340 filaments at the
+    defaults); one port across the ends."""
+    g = Geometry()
+    previous = g.add_node(Node(0.0, 0.0, 0.0))
+    first = previous
+
+    def add(x: float, y: float) -> None:
+        nonlocal previous
+        node = g.add_node(Node(x, y, 0.0))
+        g.add_segment(SegmentDef(previous, node, width, thickness, COPPER))
+        previous = node
+
+    step_len = run_length / segments_per_run
+    for run in range(runs):
+        y = run * pitch
+        for step in range(1, segments_per_run + 1):
+            along = step if run % 2 == 0 else segments_per_run - step
+            add(along * step_len, y)
+        x = (segments_per_run if run % 2 == 0 else 0) * step_len
+        add(x, y + pitch)
+    return g, [Port(first, previous, "trace")], Discretization.graded(nw, nh, ratio)
+
+
 def serpentine(bars: int) -> Tuple[Geometry, Port]:
     """benches/support.rs:64-83"""
     length, pitch, width, thickness = 2e-3, 60e-6, 20e-6, 10e-6

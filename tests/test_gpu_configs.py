@@ -119,6 +119,36 @@ def test_headline_matrix_free_agrees_with_dense(headline):
 
 
 # --------------------------------------------------------------------------
+# The 2 400-filament scaling fixture and its trace variant
+
+C2_FREQS = fixtures.log_frequencies(1e4, 1e9, 21)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("name", ["meander24", "trace_meander"])
+def test_c2_sweeps_match_the_oracle(name):
+    """meander(24) at 2 x 2 (2 400 filaments, the README's scaling fixture)
+    and the 0.2 mm x 35 um trace of 12 runs of 8 mm at 0.4 mm pitch, graded
+    5 x 3 (2 340 filaments): FP64 GPU within 1e-12 of the oracle's LU + Schur
+    at five frequencies, the refined form within 1e-10 of FP64 at all 21."""
+    prob = fixtures.meander_problem(24) if name == "meander24" else fixtures.trace_meander()
+    packed = model.pack(*prob)
+    assert packed.filament_count == (2400 if name == "meander24" else 2340)
+    system = MeshSystem.assemble_packed(packed, prob[1], "fp64")
+    z64 = system.sweep(C2_FREQS).impedance_ohm
+    system.set_precision("fp32_refined")
+    zr = system.sweep(C2_FREQS).impedance_ohm
+    system.close()
+    sub = [0, 5, 10, 15, 20]
+    zo = oracle.System(packed).sweep(C2_FREQS[sub])
+    for j, k in enumerate(sub):
+        assert rel_frobenius(z64[k], zo[j]) <= 1e-12, (name, k)
+    err = max(rel_frobenius(zr[k], z64[k]) for k in range(len(C2_FREQS)))
+    print(f"{name}: refined vs fp64 {err:.2e}", flush=True)
+    assert err <= 1e-10
+
+
+# --------------------------------------------------------------------------
 # The batched sweep at full size
 
 BATCH_FREQS = fixtures.log_frequencies(1e5, 3e9, 16)
