@@ -382,8 +382,11 @@ def main() -> int:
                     help="leave the adjoint gradients out of the headline line")
     ap.add_argument("--no-fd", action="store_true",
                     help="adjoint workload: skip the central-difference check")
-    ap.add_argument("--sweep-parts", type=int, default=2, choices=[1, 2, 3, 4],
-                    help="refined sweep: groups of frequencies factorized side by side on as many streams")
+    ap.add_argument("--sweep-parts", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="refined sweep: groups of frequencies factorized side by side on as many "
+                         "streams (0: the autotuner's choice)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default schedule (outer block 256, two sweep parts)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter scale for rocprofv3)")
     args = ap.parse_args()
@@ -418,10 +421,19 @@ def main() -> int:
     t0 = time.perf_counter()
     system = MeshSystem.assemble_packed(packed, ports, args.precision)
     assembly_s = time.perf_counter() - t0
-    system.set_sweep_parts(args.sweep_parts)
     counts = system.counts()
     print(f"# assembled in {assembly_s:.2f} s: m={counts.internal_meshes} p={counts.ports}",
           flush=True)
+    tuned, tune_s = None, 0.0
+    if args.sweep_parts:
+        system.set_schedule(system.schedule()["outer_block"], args.sweep_parts)
+    elif not args.no_autotune:
+        # the sweep schedule (outer block x sweep parts) from the project's
+        # autotuner, timed on this system's own sweep before the warm-up
+        t0 = time.perf_counter()
+        tuned, best_ms = system.autotune(freqs)
+        tune_s = time.perf_counter() - t0
+        print(f"# autotuned schedule {tuned} ({best_ms:.1f} ms per sweep) in {tune_s:.1f} s", flush=True)
 
     for w in range(args.warmup):
         system.sweep_raw(freqs)
@@ -522,21 +534,24 @@ def main() -> int:
                    "factor_arith": {0: "fp64", 1: "fp32-mfma", 2: "fp32 split-bf16 mfma"}[
                        infos[-1]["factor_arith"]],
                    "refine_steps_max": infos[-1]["refine_steps_max"],
-                   "refine_change_max": infos[-1]["refine_change_max"]},
+                   "refine_change_max": infos[-1]["refine_change_max"],
+                   "schedule": system.schedule(), "autotuned": tuned is not None,
+                   "autotune_s": tune_s},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "flops": "algorithmic: 8 real flops per complex multiply-add of the update "
                               "(FP32-equivalent), (4/3) m^3 per frequency over all launches",
                      "traffic": UPDATE_TRAFFIC_BYTES if (args.bars, args.freqs, args.precision,
-                                                         args.sweep_parts) == (49, 41, "fp32_refined", 2)
-                     else None,
+                                                         system.schedule()["sweep_parts"],
+                                                         system.schedule()["outer_block"]) ==
+                     (49, 41, "fp32_refined", 2, 256) else None,
                      "kernel": "outer trailing update of the L D L^T factorization "
                                f"({peak_kind} matrix cores)",
                      "matrix_core_tflops": unit_rate,
                      "matrix_core_frac": unit_rate / peak,
                      "launches": upd_n, "avg_launch_ms": upd_ms / max(upd_n, 1),
                      "flops_per_launch": upd_flops / max(upd_n, 1),
-                     "streams": args.sweep_parts if args.precision != "fp64" else 1,
+                     "streams": system.schedule()["sweep_parts"] if args.precision != "fp64" else 1,
                      # with the parts' launches overlapping, a launch's own
                      # duration stretches; this is every update flop of the
                      # timed steps over their whole wall time

@@ -18,6 +18,6 @@ from .api import (ConductorPeak, Counts, CurrentSolution, FilamentCurrent,  # no
                   Timing, device_math, discretize_geometry, discretize_packed, discretize_tangent,
                   gmres_dense, iterative_params, partial_inductance_derivative,
                   partial_inductance_matrix, partial_inductance_matrix_masked, pfft_params, solve,
-                  solve_batch, spice_subckt,
+                  solve_batch, spice_subckt, tune_load, tune_save,
                   subckt_name_from_path, version, zc_mat)
 from . import deck  # noqa: E402,F401

@@ -400,6 +400,28 @@ class MeshSystem:
     def set_profiling(self, enable: bool) -> None:
         native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
 
+    def schedule(self) -> dict:
+        """The sweep's schedule (fh_system_get_schedule)."""
+        sc = native.Schedule()
+        native.check(native.lib().fh_system_get_schedule(self._h, C.byref(sc)))
+        return sc.as_dict()
+
+    def set_schedule(self, outer_block: int, sweep_parts: int) -> None:
+        """Fix the sweep's schedule (fh_system_set_schedule)."""
+        sc = native.Schedule(int(outer_block), int(sweep_parts))
+        native.check(native.lib().fh_system_set_schedule(self._h, C.byref(sc)))
+
+    def autotune(self, frequencies) -> tuple:
+        """Time the candidate schedules on this system's sweep of
+        ``frequencies``, keep the fastest (fh_system_autotune).  Returns
+        (schedule dict, its sweep time in ms)."""
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        sc = native.Schedule()
+        ms = C.c_double()
+        native.check(native.lib().fh_system_autotune(self._h, native.ptr(f), len(f), C.byref(sc),
+                                                     C.byref(ms)))
+        return sc.as_dict(), ms.value
+
     def set_refinement(self, stages: int) -> None:
         """The refinement stages the refined form may use (a mask of
         native.FH_REFINE_*; fh_system_set_refinement)."""
@@ -707,6 +729,16 @@ def solve_batch(problems, frequencies, solver: SolverChoice = SolverChoice.DENSE
         out.append(z[o:o + K * p * p].reshape(K, p, p))
         o += K * p * p
     return out, [i.as_dict() for i in infos]
+
+
+def tune_save(path: str) -> None:
+    """Write the tuning cache to a JSON file (fh_tune_save)."""
+    native.check(native.lib().fh_tune_save(str(path).encode()))
+
+
+def tune_load(path: str) -> None:
+    """Merge a tuning cache file into this process's cache (fh_tune_load)."""
+    native.check(native.lib().fh_tune_load(str(path).encode()))
 
 
 def discretize_geometry(geometry: Geometry, discretization: Discretization) -> np.ndarray:

@@ -48,4 +48,17 @@ double rust_powi(double a, int b);
 fh_status discretize_tangent(const fh_problem &p, const double *dn, const double *dw, const double *dh,
                              std::vector<fh_filament> &out);
 
+// The tuning cache (fh_tune.cpp): schedules chosen by timing candidates,
+// keyed by size class, kept for the process and in a JSON file on request.
+struct TunedSchedule {
+  int outer_block = 0;
+  int sweep_parts = 0;
+  double ms = 0.0;
+};
+std::string dense_tune_key(bool refined, size_t internal_loops, size_t frequencies);
+bool tune_lookup(const std::string &key, TunedSchedule &out);
+void tune_store(const std::string &key, const TunedSchedule &value);
+fh_status tune_load(const char *path);
+fh_status tune_save(const char *path);
+
 }  // namespace fh

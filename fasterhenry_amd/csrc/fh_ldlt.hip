@@ -1563,12 +1563,14 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
 
 }  // namespace
 
-size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined) {
+int ldlt_default_nb(bool refined) { return refined ? kNbRefined : kNbPlain; }
+
+size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined, int nb) {
   DenseSolveInputs in;
   in.m = m;
   in.p = p;
-  if (!refined) return factor_bytes<double>(layout_of(in, kNbPlain), 1);
-  const Layout L = layout_of(in, kNbRefined);
+  if (!refined) return factor_bytes<double>(layout_of(in, nb > 0 ? nb : kNbPlain), 1);
+  const Layout L = layout_of(in, nb > 0 ? nb : kNbRefined);
   // the FP64 vectors of the refinement: X, R, DX, P, W, up to 16 partial
   // sums of L X and the GMRES-IR basis, 2 p rows of m each
   return factor_bytes<float>(L, 1) +
@@ -1579,7 +1581,7 @@ size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined) {
 fh_status ldlt_solve_batch(const DenseSolveInputs &in, const double *d_omegas, int K, double *d_z,
                            BatchResult &out, LdltWorkspace &ws, hipStream_t stream,
                            Profiler *prof) {
-  const Layout L = layout_of(in, kNbPlain);
+  const Layout L = layout_of(in, in.nb > 0 ? in.nb : kNbPlain);
   fh_status st;
   const size_t fbytes = factor_bytes<double>(L, K);
   if ((st = ws.reserve(fbytes + static_cast<size_t>(K) * sizeof(int)))) return st;
@@ -1604,7 +1606,7 @@ fh_status ldlt_solve_batch(const DenseSolveInputs &in, const double *d_omegas, i
 fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas, int K,
                              double *d_z, BatchResult &out, LdltWorkspace &ws,
                              hipStream_t stream, Profiler *prof) {
-  const Layout L = layout_of(in, kNbRefined);
+  const Layout L = layout_of(in, in.nb > 0 ? in.nb : kNbRefined);
   const int m = L.m, p = L.p, V = K * p, Q = 2 * V;
   fh_status st;
   const size_t fbytes = factor_bytes<float>(L, K);

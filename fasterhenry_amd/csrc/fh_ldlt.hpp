@@ -34,6 +34,9 @@ struct DenseSolveInputs {
   // with classical refinement, 1 the same factor under GMRES-IR, 2 the FP32
   // factor with classical refinement, 3 the FP32 factor under GMRES-IR)
   unsigned refine_stages = 15u;
+  // outer block of the factorization (columns), 0 = the form's default
+  // (128 FP64, 256 refined); a multiple of 64
+  int nb = 0;
 };
 
 // Grow-only device workspace (through device_alloc, so that a batch
@@ -123,7 +126,10 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
                              double *d_z, BatchResult &out, LdltWorkspace &ws,
                              hipStream_t stream, Profiler *prof);
 
-// Device bytes one frequency of a batch needs.
-size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined);
+// Device bytes one frequency of a batch needs (nb: the outer block, 0 =
+// the default).
+size_t ldlt_bytes_per_frequency(size_t m, size_t p, bool refined, int nb = 0);
+// The default outer block of a form.
+int ldlt_default_nb(bool refined);
 
 }  // namespace fh

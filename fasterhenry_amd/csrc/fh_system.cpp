@@ -136,6 +136,11 @@ struct fh_system {
   static constexpr int kParts = kSweepPartsMax;
   int sweep_parts = 2;  // two measured fastest on MI355X (four contend)
   uint32_t refine_stages = FH_REFINE_ALL;
+  // outer block of the factorization per form (0: the default), and whether
+  // the schedule was set explicitly (then the tuning cache is not consulted)
+  int nb_refined = 0, nb_plain = 0;
+  bool schedule_fixed = false;
+  int nb_for(bool refined) const { return refined ? nb_refined : nb_plain; }
   hipStream_t part_stream[kParts] = {nullptr, nullptr, nullptr, nullptr};
   LdltWorkspace part_ws[kParts];
   double inductance_s = 0.0, assembly_s = 0.0;
@@ -247,6 +252,20 @@ fh_status upload_mask(const fh_pair_mask *mask, DeviceArray<int32_t> &d_group, D
   return FH_OK;
 }
 
+// The tuning cache's schedule for this system's size class, unless the
+// caller fixed one (fh_system_set_schedule).
+void apply_tuned_schedule(fh_system *s, bool refined, size_t K) {
+  if (s->schedule_fixed) return;
+  TunedSchedule t;
+  if (!tune_lookup(dense_tune_key(refined, s->basis.internal, K), t)) return;
+  if (refined) {
+    s->nb_refined = t.outer_block;
+    if (t.sweep_parts >= 1 && t.sweep_parts <= kSweepPartsMax) s->sweep_parts = t.sweep_parts;
+  } else {
+    s->nb_plain = t.outer_block;
+  }
+}
+
 // The frequencies of a refined chunk that no FP32 stage accepted, solved by
 // the plain FP64 form (ldlt_solve_batch) in groups that fit the memory
 // budget; their Z goes to z_chunk (kc x p x p {re, im}, host) and res marks
@@ -258,7 +277,9 @@ fh_status fp64_resolve(fh_system *s, const DenseSolveInputs &in, const double *o
     if (!res.converged[q]) left.push_back(q);
   if (left.empty()) return FH_OK;
   const size_t p = in.p;
-  const size_t per = ldlt_bytes_per_frequency(in.m, p, false);
+  DenseSolveInputs in64 = in;
+  in64.nb = s->nb_plain;
+  const size_t per = ldlt_bytes_per_frequency(in.m, p, false, in64.nb);
   const size_t group = std::min(left.size(), std::max<size_t>(1, sweep_budget_bytes() / per));
   DeviceArray<double> d_w, d_z;
   fh_status st;
@@ -270,7 +291,7 @@ fh_status fp64_resolve(fh_system *s, const DenseSolveInputs &in, const double *o
     for (size_t k = 0; k < gn; ++k) w[k] = omegas[left[g0 + k]];
     FH_HIP_TRY(hipMemcpyAsync(d_w.ptr, w.data(), gn * sizeof(double), hipMemcpyHostToDevice, s->stream));
     BatchResult r64;
-    if ((st = ldlt_solve_batch(in, d_w.ptr, static_cast<int>(gn), d_z.ptr, r64, ws, s->stream, nullptr)))
+    if ((st = ldlt_solve_batch(in64, d_w.ptr, static_cast<int>(gn), d_z.ptr, r64, ws, s->stream, nullptr)))
       return st;
     FH_HIP_TRY(hipMemcpyAsync(z.data(), d_z.ptr, gn * p * p * 2 * sizeof(double), hipMemcpyDeviceToHost,
                               s->stream));
@@ -294,7 +315,7 @@ fh_status refined_x(fh_system *s, const double *freqs, size_t K, std::vector<dou
   const size_t m = s->basis.internal, p = s->basis.loops - m;
   X.assign(2 * K * p * m, 0.0);
   if (m == 0 || K == 0) return FH_OK;
-  const size_t per = ldlt_bytes_per_frequency(m, p, true) + 2 * p * m * sizeof(double);
+  const size_t per = ldlt_bytes_per_frequency(m, p, true, s->nb_refined) + 2 * p * m * sizeof(double);
   const size_t chunk = std::min(K, std::max<size_t>(1, sweep_budget_bytes() / per));
   DeviceArray<double> d_omegas, d_z, d_x;
   fh_status st;
@@ -317,6 +338,7 @@ fh_status refined_x(fh_system *s, const double *freqs, size_t K, std::vector<dou
   in.d_x_out = d_x.ptr;
   in.tight_x = true;
   in.refine_stages = s->refine_stages & 15u;
+  in.nb = s->nb_refined;
   std::vector<double> omegas(chunk), z(2 * chunk * p * p);
   for (size_t c0 = 0; c0 < K; c0 += chunk) {
     const size_t kc = std::min(chunk, K - c0);
@@ -724,7 +746,8 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     }
   } else {
     const bool refined = s->precision == FH_FP32_REFINED;
-    const size_t per = ldlt_bytes_per_frequency(m, p, refined);
+    apply_tuned_schedule(s, refined, K);
+    const size_t per = ldlt_bytes_per_frequency(m, p, refined, s->nb_for(refined));
     size_t chunk = std::max<size_t>(1, sweep_budget_bytes() / per);
     chunk = std::min(chunk, K);
     DeviceArray<double> d_omegas, d_z;
@@ -746,6 +769,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     in.d_rep = s->d_rep.ptr;
     in.d_rpp = s->d_rpp.ptr;
     in.refine_stages = s->refine_stages & 15u;
+    in.nb = s->nb_for(refined);
     std::vector<double> omegas(chunk);
     for (size_t c0 = 0; c0 < K; c0 += chunk) {
       const size_t kc = std::min(chunk, K - c0);
@@ -897,6 +921,91 @@ fh_status fh_system_set_refinement(fh_system *s, uint32_t stages) {
   if (stages == 0 || (stages & ~FH_REFINE_ALL))
     return set_error(FH_E_INVALID_ARGUMENT, "stages must be a non-empty mask of FH_REFINE_* bits");
   s->refine_stages = stages;
+  return FH_OK;
+}
+
+fh_status fh_system_get_schedule(const fh_system *s, fh_schedule *out) {
+  clear_error();
+  if (!s || !out) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  const bool refined = s->precision == FH_FP32_REFINED;
+  const int nb = s->nb_for(refined);
+  out->outer_block = nb > 0 ? nb : ldlt_default_nb(refined);
+  out->sweep_parts = s->sweep_parts;
+  return FH_OK;
+}
+
+fh_status fh_system_set_schedule(fh_system *s, const fh_schedule *sched) {
+  clear_error();
+  if (!s || !sched) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (sched->outer_block < 64 || sched->outer_block > 1024 || sched->outer_block % 64 != 0)
+    return set_error(FH_E_INVALID_ARGUMENT, "outer_block must be a multiple of 64 from 64 to 1024");
+  if (sched->sweep_parts < 1 || sched->sweep_parts > kSweepPartsMax)
+    return set_error(FH_E_INVALID_ARGUMENT, "sweep_parts must be 1 to 4");
+  if (s->precision == FH_FP32_REFINED)
+    s->nb_refined = sched->outer_block;
+  else
+    s->nb_plain = sched->outer_block;
+  s->sweep_parts = sched->sweep_parts;
+  s->schedule_fixed = true;
+  return FH_OK;
+}
+
+fh_status fh_system_autotune(fh_system *s, const double *freqs, size_t K, fh_schedule *chosen,
+                             double *best_ms) {
+  clear_error();
+  if (!s || (!freqs && K)) return set_error(FH_E_INVALID_ARGUMENT, "null argument");
+  if (s->solver == FH_SOLVER_ITERATIVE || s->basis.internal == 0 || K == 0) {
+    if (chosen) (void)fh_system_get_schedule(s, chosen);
+    if (best_ms) *best_ms = 0.0;
+    return FH_OK;  // nothing to tune: no dense factorization runs
+  }
+  const bool refined = s->precision == FH_FP32_REFINED;
+  const size_t p = s->basis.loops - s->basis.internal;
+  std::vector<double> z(2 * K * p * p);
+  std::vector<fh_schedule> cand;
+  for (const int nb : refined ? std::vector<int>{512, 256, 128} : std::vector<int>{256, 128, 64})
+    for (int parts = refined ? 3 : 1; parts >= 1; --parts) cand.push_back(fh_schedule{nb, parts});
+  const bool was_fixed = s->schedule_fixed;
+  const int nb0 = s->nb_for(refined), parts0 = s->sweep_parts;
+  fh_status st;
+  // the first candidate needs the most workspace: grow it once, untimed
+  s->schedule_fixed = true;
+  s->nb_refined = refined ? cand[0].outer_block : s->nb_refined;
+  s->nb_plain = refined ? s->nb_plain : cand[0].outer_block;
+  s->sweep_parts = cand[0].sweep_parts;
+  if ((st = fh_sweep(s, freqs, K, z.data(), nullptr))) {
+    s->schedule_fixed = was_fixed;
+    return st;
+  }
+  double best = INFINITY;
+  fh_schedule win{nb0 > 0 ? nb0 : ldlt_default_nb(refined), parts0};
+  for (const fh_schedule &c : cand) {
+    if (refined)
+      s->nb_refined = c.outer_block;
+    else
+      s->nb_plain = c.outer_block;
+    s->sweep_parts = c.sweep_parts;
+    const auto t0 = Clock::now();
+    if ((st = fh_sweep(s, freqs, K, z.data(), nullptr))) {
+      s->schedule_fixed = was_fixed;
+      return st;
+    }
+    const double ms = seconds_since(t0) * 1e3;
+    if (ms < best) {
+      best = ms;
+      win = c;
+    }
+  }
+  if (refined)
+    s->nb_refined = win.outer_block;
+  else
+    s->nb_plain = win.outer_block;
+  s->sweep_parts = win.sweep_parts;
+  s->schedule_fixed = was_fixed;
+  tune_store(dense_tune_key(refined, s->basis.internal, K),
+             TunedSchedule{win.outer_block, win.sweep_parts, best});
+  if (chosen) *chosen = win;
+  if (best_ms) *best_ms = best;
   return FH_OK;
 }
 

@@ -41,6 +41,8 @@ SYMBOLS = [
     "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
     "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
     "fh_system_set_sweep_parts", "fh_partial_inductance_noalloc", "fh_system_set_refinement",
+    "fh_system_get_schedule", "fh_system_set_schedule", "fh_system_autotune", "fh_tune_save",
+    "fh_tune_load",
 ]
 
 _ERRORS = {
@@ -73,6 +75,14 @@ class SweepInfo(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Schedule(C.Structure):
+    """fh_schedule: outer block of the factorization and sweep parts."""
+    _fields_ = [("outer_block", C.c_int32), ("sweep_parts", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"outer_block": self.outer_block, "sweep_parts": self.sweep_parts}
 
 
 class PairMask(C.Structure):
@@ -175,6 +185,16 @@ def lib() -> C.CDLL:
                                C.POINTER(SweepInfo)]
         L.fh_partial_inductance.restype = st
         L.fh_partial_inductance.argtypes = [vp, sz, vp, vp]
+        L.fh_system_get_schedule.restype = st
+        L.fh_system_get_schedule.argtypes = [vp, C.POINTER(Schedule)]
+        L.fh_system_set_schedule.restype = st
+        L.fh_system_set_schedule.argtypes = [vp, C.POINTER(Schedule)]
+        L.fh_system_autotune.restype = st
+        L.fh_system_autotune.argtypes = [vp, vp, sz, C.POINTER(Schedule), C.POINTER(C.c_double)]
+        L.fh_tune_save.restype = st
+        L.fh_tune_save.argtypes = [C.c_char_p]
+        L.fh_tune_load.restype = st
+        L.fh_tune_load.argtypes = [C.c_char_p]
         L.fh_system_set_refinement.restype = st
         L.fh_system_set_refinement.argtypes = [vp, C.c_uint32]
         L.fh_partial_inductance_noalloc.restype = st

@@ -270,6 +270,30 @@ fh_status fh_system_set_sweep_parts(fh_system *system, int32_t parts);
 #define FH_REFINE_ALL 31u
 fh_status fh_system_set_refinement(fh_system *system, uint32_t stages);
 
+/* Schedule of a dense system's sweep (no counterpart in the reference): the
+ * outer block of the factorization (columns; a multiple of 64 from 64 to
+ * 1024; default 128 FP64, 256 refined) and the sweep parts (refined form).
+ * Setting one fixes it for the system; otherwise a schedule the tuning cache
+ * holds for the system's size class (precision, internal loops and
+ * frequencies rounded up to powers of two) is used.  Results do not depend
+ * on the schedule beyond the refinement's rounding. */
+typedef struct fh_schedule {
+  int32_t outer_block;
+  int32_t sweep_parts;
+} fh_schedule;
+fh_status fh_system_get_schedule(const fh_system *system, fh_schedule *out);
+fh_status fh_system_set_schedule(fh_system *system, const fh_schedule *schedule);
+/* Autotuner: sweeps `freqs` once per candidate schedule (outer blocks 128,
+ * 256, 512 x parts 1-3 refined; 64, 128, 256 FP64), keeps the fastest for the
+ * system and stores it in the tuning cache; chosen and best_ms (either may
+ * be NULL) receive it and its sweep time.  A system with no dense
+ * factorization has nothing to tune. */
+fh_status fh_system_autotune(fh_system *system, const double *freqs, size_t K, fh_schedule *chosen,
+                             double *best_ms);
+/* The tuning cache to and from a JSON file (version 1). */
+fh_status fh_tune_save(const char *path);
+fh_status fh_tune_load(const char *path);
+
 /* Dense systems only (FH_E_UNSUPPORTED for an iterative one). */
 fh_status fh_system_inductance(const fh_system *system, double *L_out);
 fh_status fh_system_blocks(const fh_system *system, double *r_ee, double *l_ee, double *r_ep,

@@ -232,6 +232,37 @@ void geometry() {
   check(fh_discretize(&q, nullptr, 0, &n) != FH_OK, "zero-length segment rejected");
 }
 
+void tuning_cache(const std::string &tmp) {
+  fh::tune_store(fh::dense_tune_key(true, 7350, 41), fh::TunedSchedule{256, 2, 86.0});
+  fh::tune_store(fh::dense_tune_key(false, 100, 3), fh::TunedSchedule{128, 1, 1.0});
+  check(fh::dense_tune_key(true, 7350, 41) == "dense/refined/m8192/k64", "tune key");
+  check(fh_tune_save(tmp.c_str()) == FH_OK, "tune save");
+  check(fh_tune_load(tmp.c_str()) == FH_OK, "tune load");
+  fh::TunedSchedule t;
+  check(fh::tune_lookup("dense/refined/m8192/k64", t) && t.outer_block == 256 && t.sweep_parts == 2,
+        "tune round trip");
+  const std::string text = read_file(tmp);
+  std::mt19937_64 rng(3);
+  for (int trial = 0; trial < 300; ++trial) {
+    std::string bad = text;
+    for (int e = 0; e < 3 && !bad.empty(); ++e) {
+      const size_t at = rng() % bad.size();
+      if (rng() % 2) bad.erase(at, 1 + rng() % 8);
+      else {
+        static const char junk[] = "{}[]\":,0123456789-e";
+        bad[at] = junk[rng() % (sizeof junk - 1)];
+      }
+    }
+    {
+      std::ofstream f(tmp);
+      f << bad;
+    }
+    (void)fh_tune_load(tmp.c_str());
+  }
+  check(fh_tune_load("/nonexistent/dir/tune.json") != FH_OK, "missing tuning file");
+  std::remove(tmp.c_str());
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -239,6 +270,7 @@ int main(int argc, char **argv) {
   decks(dir);
   writers();
   geometry();
+  tuning_cache(argc > 2 ? argv[2] : "/tmp/fh_host_driver_tune.json");
   if (failures) {
     std::fprintf(stderr, "%d checks failed\n", failures);
     return 1;
