@@ -400,6 +400,10 @@ class MeshSystem:
     def set_profiling(self, enable: bool) -> None:
         native.check(native.lib().fh_system_set_profiling(self._h, 1 if enable else 0))
 
+    def set_graphs(self, enable: bool) -> None:
+        """Graph replay of repeated refined sweeps (fh_system_set_graphs)."""
+        native.check(native.lib().fh_system_set_graphs(self._h, 1 if enable else 0))
+
     def schedule(self) -> dict:
         """The sweep's schedule (fh_system_get_schedule)."""
         sc = native.Schedule()

@@ -65,6 +65,7 @@ constexpr int kMaxRefine = 30;
 constexpr int kMaxGmresRefine = 8;   // outer steps under GMRES-IR
 constexpr int kGmresRestart = 20;    // inner GMRES steps per outer step
 constexpr double kGmresTol = 1e-8;   // inner reduction of the preconditioned residual
+constexpr int kGraphSteps = 2;       // classical steps inside the first factorization's graph
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -1344,6 +1345,73 @@ __global__ void reset_vectors_kernel(int m, int p, int V, const int *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Device-side state of the captured refinement steps: per frequency f,
+// st[0 K + f] converged (0/1), st[1 K + f] steps, st[2 K + f] the last
+// first-order term |x^T r| / |Z|, st[3 K + f] its value one step earlier,
+// st[4 K + f] |dx| one step earlier, st[5 K + f] the first |dx|.
+
+__global__ void fill_kernel(int64_t n, double *__restrict__ x, double v) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+
+__global__ void fill_int_kernel(int n, int *__restrict__ x, int v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+
+__global__ void refine_state_init_kernel(int K, double *__restrict__ st) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K) return;
+  st[f] = 0.0;
+  st[K + f] = 0.0;
+  st[2 * K + f] = INFINITY;
+  st[3 * K + f] = INFINITY;
+  st[4 * K + f] = 0.0;
+  st[5 * K + f] = 0.0;
+}
+
+// The acceptance test of step `it` of the classical refinement, the host's
+// test of ldlt_refined_batch operation for operation (same sums in the same
+// order), for the frequencies not yet converged.
+__global__ void refine_accept_kernel(int K, int p, int it, const double *__restrict__ metric,
+                                     const double *__restrict__ nrm, const double *__restrict__ z,
+                                     const int *__restrict__ sing, double z_tol, double tol,
+                                     double *__restrict__ st) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= K || st[f] != 0.0) return;
+  double worst = 0.0;
+  for (int q = 0; q < p * p; ++q) worst = fmax(worst, metric[f * p * p + q]);
+  if (!(worst == worst)) worst = INFINITY;
+  st[K + f] += 1.0;
+  st[2 * K + f] = worst;
+  double sdx = 0.0, sr = 0.0, sz = 0.0;
+  for (int q = 0; q < p; ++q) {
+    sdx += nrm[2 * (f * p + q)];
+    sr += nrm[2 * (f * p + q) + 1];
+  }
+  for (int q = 0; q < 2 * p * p; ++q) {
+    const double zq = z[2 * f * p * p + q];
+    sz += zq * zq;
+  }
+  const double ndx = sqrt(sdx), nr = sqrt(sr), nz = sqrt(sz);
+  double bound = INFINITY;
+  if (st[5 * K + f] == 0.0 && isfinite(ndx)) st[5 * K + f] = ndx;
+  const double pdx = st[4 * K + f];
+  if (it >= 2 && pdx > 0.0) {
+    const double rho = ndx / pdx;
+    const double x_bound = rho <= 0.5 ? rho / (1.0 - rho) * ndx : INFINITY;
+    if (rho <= 0.5 && nz > 0.0) bound = x_bound * nr / nz;
+  }
+  st[4 * K + f] = ndx;
+  const double prev = st[3 * K + f];
+  const bool done = isfinite(nz) && sing[f] == 0 &&
+                    (bound <= z_tol || worst <= tol || (worst <= 1e-9 && !(worst < 0.5 * prev)));
+  if (done) st[f] = 1.0;
+  st[3 * K + f] = worst;
+}
+
+// ---------------------------------------------------------------------------
 // Host drivers.
 
 struct Layout {
@@ -1534,7 +1602,8 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
                        hipStream_t stream) {
   const int m = L.m;
   const size_t n = static_cast<size_t>(2) * V * m;
-  FH_HIP_TRY(hipMemcpyAsync(work, vec, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  axpy_copy_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
+      static_cast<int64_t>(n), nullptr, nullptr, work, vec);
   const int nsb = (m + kSB - 1) / kSB;
   for (int sb = 0; sb < nsb; ++sb) {
     const int B0 = sb * kSB, bw = std::min(kSB, m - B0);
@@ -1622,6 +1691,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
                        static_cast<size_t>(V) * sizeof(double) +         // vnrm
                        4 * static_cast<size_t>(V) * hld * sizeof(double) +  // h (2 passes)
                        2 * static_cast<size_t>(V) * hld * sizeof(double) +  // y
+                       6 * static_cast<size_t>(K) * sizeof(double) +        // device refinement state
                        2 * static_cast<size_t>(K) * sizeof(int) + 256;      // sing, reset
   const size_t extra = (5 + static_cast<size_t>(lx_slices) + hld) * vbytes + small;
   const size_t sbytes = static_cast<size_t>(split_blocks(L)) * 4096 * K;
@@ -1649,7 +1719,8 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   double *d_vnrm = d_nrm + 2 * static_cast<size_t>(V);
   double *d_h = d_vnrm + V;
   double *d_y = d_h + 4 * static_cast<size_t>(V) * hld;
-  int *d_sing = reinterpret_cast<int *>(d_y + 2 * static_cast<size_t>(V) * hld);
+  double *d_state = d_y + 2 * static_cast<size_t>(V) * hld;
+  int *d_sing = reinterpret_cast<int *>(d_state + 6 * static_cast<size_t>(K));
   int *d_reset = d_sing + K;
   const unsigned vec_blocks = static_cast<unsigned>((n + 255) / 256);
   const unsigned res_blocks = static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256);
@@ -1803,10 +1874,16 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
 
   // Refinement steps from the current X with the current factor, classical
   // (dx = M^-1 r) or GMRES-IR; marks the frequencies that converge.
-  auto refine = [&](bool gmres, int stage, int max_steps) -> fh_status {
-    std::vector<double> prev(K, INFINITY), prev_dx(K, 0.0);
+  // prev / prev_dx: the first-order term and |dx| one step earlier, carried
+  // into a stage that continues after `first_it - 1` steps already taken.
+  std::vector<double> prev, prev_dx;
+  auto refine = [&](bool gmres, int stage, int max_steps, int first_it) -> fh_status {
+    if (first_it == 1) {
+      prev.assign(K, INFINITY);
+      prev_dx.assign(K, 0.0);
+    }
     fh_status s2;
-    for (int it = 1; it <= max_steps; ++it) {
+    for (int it = first_it; it <= max_steps; ++it) {
       if (gmres) {
         int inner = 0;
         if ((s2 = gmres_correction(inner))) return s2;
@@ -1895,6 +1972,99 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   const bool split_ok = split_blocks(L) * 4096 < (int64_t{1} << 31);
   int factored = -1;  // 0 split, 1 FP32
   bool started = false;
+  int stage0_from = 1;  // first classical step of stage 0 still to run on the host
+  const unsigned kb = static_cast<unsigned>((K + 255) / 256);
+  // The split factorization and its first kGraphSteps classical steps with
+  // the acceptance test on the device: enqueued as they are, or captured
+  // into the chunk's graph and replayed.
+  auto stage0_steps = [&](Profiler *pr) -> fh_status {
+    fill_int_kernel<<<kb, 256, 0, stream>>>(K, d_sing, 0);
+    fh_status s2;
+    if ((s2 = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, pr, &sb))) return s2;
+    fill_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, 0.0);
+    refine_state_init_kernel<<<kb, 256, 0, stream>>>(K, d_state);
+    if ((s2 = residual())) return s2;
+    for (int it = 1; it <= kGraphSteps; ++it) {
+      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
+      if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
+      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
+      if ((s2 = residual())) return s2;
+      functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
+                                                        in.d_lmesh, lds, X, R, d_z, metric);
+      norms_kernel<<<V, 256, 0, stream>>>(m, DX, R, d_nrm);
+      refine_accept_kernel<<<kb, 256, 0, stream>>>(K, p, it, metric, d_nrm, d_z, d_sing, kZTol, kTol,
+                                                   d_state);
+      FH_HIP_TRY(hipGetLastError());
+    }
+    return FH_OK;
+  };
+  if (in.graph && !in.tight_x && (in.refine_stages & 1u) && split_ok) {
+    RefineGraph &gr = *in.graph;
+    const void *mats[4] = {in.d_lmesh, in.d_ree_val, in.d_rep, in.d_r_vals};
+    const bool same = gr.base == ws.base && gr.omegas == d_omegas && gr.z == d_z && gr.K == K &&
+                      gr.nb == L.nb && gr.profiled == (prof != nullptr) &&
+                      std::equal(mats, mats + 4, gr.mats);
+    if (!same) {
+      gr.reset();
+      std::copy(mats, mats + 4, gr.mats);
+      gr.base = ws.base;
+      gr.omegas = d_omegas;
+      gr.z = d_z;
+      gr.K = K;
+      gr.nb = L.nb;
+      gr.profiled = prof != nullptr;
+    }
+    ++gr.seen;
+    if (gr.exec || gr.seen >= 2) {
+      if (!gr.exec) {
+        FH_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        gr.prof.captured = true;
+        st = stage0_steps(gr.profiled ? &gr.prof : nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (st) {
+          if (g) (void)hipGraphDestroy(g);
+          gr.reset();
+          return st;
+        }
+        if (e != hipSuccess || !g) {
+          gr.reset();
+          return set_error(FH_E_HIP, std::string("graph capture of the refined solve failed: ") +
+                                         hipGetErrorString(e));
+        }
+        gr.graph = g;
+        FH_HIP_TRY(hipGraphInstantiate(&gr.exec, gr.graph, nullptr, nullptr, 0));
+      }
+      FH_HIP_TRY(hipGraphLaunch(gr.exec, stream));
+    } else {
+      if ((st = stage0_steps(prof))) return st;
+    }
+  } else if ((in.refine_stages & 1u) && split_ok && !in.tight_x) {
+    if ((st = stage0_steps(prof))) return st;
+  }
+  if ((in.refine_stages & 1u) && split_ok && !in.tight_x) {
+    // the device state of the first steps, into the host's bookkeeping
+    std::vector<double> hs(6 * static_cast<size_t>(K));
+    FH_HIP_TRY(hipMemcpyAsync(hs.data(), d_state, hs.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    FH_HIP_TRY(hipMemcpyAsync(h_sing.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost, stream));
+    FH_HIP_TRY(hipStreamSynchronize(stream));
+    if (in.graph && in.graph->exec && prof) prof->absorb(in.graph->prof);
+    prev.assign(K, INFINITY);
+    prev_dx.assign(K, 0.0);
+    for (int f = 0; f < K; ++f) {
+      out.converged[f] = hs[f] != 0.0 ? 1 : 0;
+      out.stage[f] = out.converged[f] ? 0 : -1;
+      out.refine_steps[f] = static_cast<int>(hs[K + f]);
+      out.refine_change[f] = hs[2 * K + f];
+      prev[f] = hs[3 * K + f];
+      prev_dx[f] = hs[4 * K + f];
+      first_dx[f] = hs[5 * K + f];
+    }
+    factored = 0;
+    started = true;
+    out.factor_arith = 2;
+    stage0_from = kGraphSteps + 1;
+  }
   for (int stage = 0; stage < 4; ++stage) {
     if (!(in.refine_stages & (1u << stage))) continue;
     const int want = stage < 2 ? 0 : 1;
@@ -1903,14 +2073,14 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     for (int f = 0; f < K; ++f) all = all && out.converged[f];
     if (all) break;
     if (factored != want) {
-      FH_HIP_TRY(hipMemsetAsync(d_sing, 0, K * sizeof(int), stream));
+      fill_int_kernel<<<kb, 256, 0, stream>>>(K, d_sing, 0);
       if ((st = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, prof, want == 0 ? &sb : nullptr)))
         return st;
       FH_HIP_TRY(hipMemcpyAsync(h_sing.data(), d_sing, K * sizeof(int), hipMemcpyDeviceToHost, stream));
       factored = want;
       out.factor_arith = want == 0 ? 2 : 1;
       if (!started) {
-        FH_HIP_TRY(hipMemsetAsync(X, 0, n * sizeof(double), stream));
+        fill_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, 0.0);
         started = true;
       } else {
         // restart the frequencies not yet converged from x = 0: an earlier
@@ -1926,7 +2096,8 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
       FH_HIP_TRY(hipStreamSynchronize(stream));  // h_sing, h_reset
     }
     const bool gmres = stage == 1 || stage == 3;
-    if ((st = refine(gmres, stage, gmres ? kMaxGmresRefine : kMaxRefine))) return st;
+    if ((st = refine(gmres, stage, gmres ? kMaxGmresRefine : kMaxRefine, stage == 0 ? stage0_from : 1)))
+      return st;
   }
   // A zero pivot of an FP32 factor is not a singular system: it is reported
   // as not converged, for the caller to solve in FP64.

@@ -37,6 +37,8 @@ struct DenseSolveInputs {
   // outer block of the factorization (columns), 0 = the form's default
   // (128 FP64, 256 refined); a multiple of 64
   int nb = 0;
+  // refined form, may be null: the chunk's graph for replay
+  struct RefineGraph *graph = nullptr;
 };
 
 // Grow-only device workspace (through device_alloc, so that a batch
@@ -68,6 +70,9 @@ struct LdltWorkspace {
 struct Profiler {
   std::vector<hipEvent_t> starts, stops;
   size_t used = 0;
+  // recording into a captured graph: the events become event-record nodes
+  // that stamp each replay
+  bool captured = false;
   double flops = 0.0;  // algorithmic (FP32/FP64-equivalent) flops
   double unit_flops = 0.0;  // flops on the matrix-core units used (3x for split bf16)
   ~Profiler() { reset_events(); }
@@ -86,22 +91,61 @@ struct Profiler {
       starts.push_back(a);
       stops.push_back(b);
     }
-    (void)hipEventRecord(starts[used], s);
+    (void)hipEventRecordWithFlags(starts[used], s, captured ? hipEventRecordExternal : 0);
   }
   void end(hipStream_t s, double f, double unit_factor = 1.0) {
-    (void)hipEventRecord(stops[used], s);
+    (void)hipEventRecordWithFlags(stops[used], s, captured ? hipEventRecordExternal : 0);
     ++used;
     flops += f;
     unit_flops += f * unit_factor;
   }
+  // Another profiler's intervals (a replayed graph's), added in.
+  double offset_ms = 0.0;
+  size_t offset_launches = 0;
+  void absorb(const Profiler &o) {
+    offset_ms += o.total_ms();
+    offset_launches += o.used + o.offset_launches;
+    flops += o.flops;
+    unit_flops += o.unit_flops;
+  }
+  size_t launches() const { return used + offset_launches; }
   // Sum of the recorded intervals (ms); the stream must be synchronized.
   double total_ms() const {
-    double t = 0.0;
+    double t = offset_ms;
     for (size_t i = 0; i < used; ++i) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, starts[i], stops[i]) == hipSuccess) t += ms;
     }
     return t;
+  }
+};
+
+// A captured refined solve of one chunk -- the factorization and the first
+// refinement steps, with the convergence test on the device -- replayed by
+// later sweeps while the launch parameters it captured still hold.  Capture
+// starts at the second sweep with the same parameters, so one-off sweeps pay
+// nothing for it.
+struct RefineGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *base = nullptr, *omegas = nullptr, *z = nullptr;
+  const void *mats[4] = {nullptr, nullptr, nullptr, nullptr};  // L, R_ee, Z_ep, R_scatter
+  int K = 0, nb = 0;
+  bool profiled = false;
+  int seen = 0;
+  Profiler prof;  // the events of the captured update launches
+  RefineGraph() = default;
+  RefineGraph(const RefineGraph &) = delete;
+  RefineGraph &operator=(const RefineGraph &) = delete;
+  ~RefineGraph() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    prof.reset_events();
+    prof.flops = prof.unit_flops = 0.0;
+    seen = 0;
   }
 };
 

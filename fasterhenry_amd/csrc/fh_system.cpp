@@ -143,6 +143,11 @@ struct fh_system {
   int nb_for(bool refined) const { return refined ? nb_refined : nb_plain; }
   hipStream_t part_stream[kParts] = {nullptr, nullptr, nullptr, nullptr};
   LdltWorkspace part_ws[kParts];
+  // the dense sweep's frequency and Z buffers, kept between sweeps so that a
+  // repeated sweep replays each part's captured graph (RefineGraph)
+  DeviceArray<double> sweep_omegas, sweep_z;
+  RefineGraph part_graph[kParts];
+  bool graphs = true;
   double inductance_s = 0.0, assembly_s = 0.0;
   // a copy of the problem (geometry tangents for gradients)
   std::vector<double> prob_nodes;
@@ -750,9 +755,12 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     const size_t per = ldlt_bytes_per_frequency(m, p, refined, s->nb_for(refined));
     size_t chunk = std::max<size_t>(1, sweep_budget_bytes() / per);
     chunk = std::min(chunk, K);
-    DeviceArray<double> d_omegas, d_z;
+    DeviceArray<double> &d_omegas = s->sweep_omegas, &d_z = s->sweep_z;
     fh_status st;
-    if ((st = d_omegas.allocate(std::max<size_t>(chunk, 1))) ||
+    if (d_omegas.count < std::max<size_t>(chunk, 1) &&
+        (st = d_omegas.allocate(std::max<size_t>(chunk, 1))))
+      return st;
+    if (d_z.count < std::max<size_t>(chunk * p * p * 2, 1) &&
         (st = d_z.allocate(std::max<size_t>(chunk * p * p * 2, 1))))
       return st;
     DenseSolveInputs in;
@@ -801,15 +809,17 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
           hipStream_t sq = q == 0 ? s->stream : s->part_stream[q];
           LdltWorkspace &wq = q == 0 ? s->ws : s->part_ws[q];
           const size_t k0 = off[q], kn = off[q + 1] - off[q];
+          DenseSolveInputs inq = in;
+          inq.graph = s->graphs ? &s->part_graph[q] : nullptr;
           fh_status stq = q == 0 || hipSetDevice(dev) == hipSuccess
-                              ? ldlt_refined_batch(in, d_omegas.ptr + k0, static_cast<int>(kn), d_z.ptr + 2 * p * p * k0,
+                              ? ldlt_refined_batch(inq, d_omegas.ptr + k0, static_cast<int>(kn), d_z.ptr + 2 * p * p * k0,
                                                    rs[q], wq, sq, pp ? &profs[q] : nullptr)
                               : set_error(FH_E_HIP, "hipSetDevice failed in a sweep part");
           if (!stq && hipStreamSynchronize(sq) != hipSuccess)
             stq = set_error(FH_E_HIP, "hipStreamSynchronize failed on a sweep part's stream");
           if (!stq && pp) {
             ms[q] = profs[q].total_ms();
-            launches[q] = profs[q].used;
+            launches[q] = profs[q].launches();
           }
           if (stq) errs[q] = error_state();
           sts[q] = stq;
@@ -847,7 +857,9 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
           }
         }
       } else {
-        st = refined ? ldlt_refined_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
+        DenseSolveInputs in1 = in;
+        in1.graph = s->graphs && !tl_worker_stream ? &s->part_graph[0] : nullptr;
+        st = refined ? ldlt_refined_batch(in1, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
                                           s->stream, pp)
                      : ldlt_solve_batch(in, d_omegas.ptr, static_cast<int>(kc), d_z.ptr, res, s->ws,
                                         s->stream, pp);
@@ -900,7 +912,7 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     info->refine_change_max = refine_change_max;
     info->refine_stage = refine_stage;
     info->update_kernel_ms = pp ? prof.total_ms() + extra_ms : 0.0;
-    info->update_kernel_launches = pp ? static_cast<int64_t>(prof.used + extra_launches) : 0;
+    info->update_kernel_launches = pp ? static_cast<int64_t>(prof.launches() + extra_launches) : 0;
     prof.flops += extra_flops;
     prof.unit_flops += extra_unit_flops;
     info->update_kernel_flops = pp ? prof.flops : 0.0;
@@ -1006,6 +1018,15 @@ fh_status fh_system_autotune(fh_system *s, const double *freqs, size_t K, fh_sch
              TunedSchedule{win.outer_block, win.sweep_parts, best});
   if (chosen) *chosen = win;
   if (best_ms) *best_ms = best;
+  return FH_OK;
+}
+
+fh_status fh_system_set_graphs(fh_system *s, int32_t enable) {
+  clear_error();
+  if (!s) return set_error(FH_E_INVALID_ARGUMENT, "null system");
+  s->graphs = enable != 0;
+  if (!s->graphs)
+    for (auto &g : s->part_graph) g.reset();
   return FH_OK;
 }
 

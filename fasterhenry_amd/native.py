@@ -41,7 +41,7 @@ SYMBOLS = [
     "fh_partial_inductance_masked", "fh_deck_parse", "fh_deck_problem", "fh_deck_info",
     "fh_deck_coupling", "fh_deck_title", "fh_deck_port_name", "fh_deck_group_name", "fh_deck_destroy",
     "fh_system_set_sweep_parts", "fh_partial_inductance_noalloc", "fh_system_set_refinement",
-    "fh_system_get_schedule", "fh_system_set_schedule", "fh_system_autotune", "fh_tune_save",
+    "fh_system_get_schedule", "fh_system_set_graphs", "fh_system_set_schedule", "fh_system_autotune", "fh_tune_save",
     "fh_tune_load",
 ]
 
@@ -185,6 +185,8 @@ def lib() -> C.CDLL:
                                C.POINTER(SweepInfo)]
         L.fh_partial_inductance.restype = st
         L.fh_partial_inductance.argtypes = [vp, sz, vp, vp]
+        L.fh_system_set_graphs.restype = st
+        L.fh_system_set_graphs.argtypes = [vp, C.c_int32]
         L.fh_system_get_schedule.restype = st
         L.fh_system_get_schedule.argtypes = [vp, C.POINTER(Schedule)]
         L.fh_system_set_schedule.restype = st

@@ -290,6 +290,13 @@ fh_status fh_system_set_schedule(fh_system *system, const fh_schedule *schedule)
  * factorization has nothing to tune. */
 fh_status fh_system_autotune(fh_system *system, const double *freqs, size_t K, fh_schedule *chosen,
                              double *best_ms);
+/* Graph replay of repeated refined sweeps (no counterpart in the reference):
+ * from the second sweep of a chunk with the same frequency count, each
+ * sweep part's factorization and first two refinement steps, with their
+ * convergence test on the device, run as one captured HIP graph; frequencies
+ * that need more steps continue as before.  The results are the same bits
+ * either way.  Default on; 0 turns it off (and frees the graphs). */
+fh_status fh_system_set_graphs(fh_system *system, int32_t enable);
 /* The tuning cache to and from a JSON file (version 1). */
 fh_status fh_tune_save(const char *path);
 fh_status fh_tune_load(const char *path);
