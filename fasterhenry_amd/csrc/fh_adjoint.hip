@@ -346,8 +346,10 @@ __device__ __forceinline__ void rule_on(int n, int i, double a, double b, double
 // ---------------------------------------------------------------------------
 // Point quadrature (neumann.rs:160-266).
 
+// The four lane sums rotate through r0..r3 (the next term always goes to
+// r0); r_j holds lane (k + j) mod 4 after k terms, undone at the end.
 __device__ Dual inner_sum(V3 pa, const Bar &b, const int ob[3]) {
-  Dual in0 = cst(0.0), in1 = cst(0.0), in2 = cst(0.0), in3 = cst(0.0);
+  Dual r0 = cst(0.0), r1 = cst(0.0), r2 = cst(0.0), r3 = cst(0.0);
   int k = 0;
   const Dual s1 = 2.0 * b.half[1], s2 = 2.0 * b.half[2];
   for (int il = 0; il < ob[0]; ++il) {
@@ -366,14 +368,20 @@ __device__ Dual inner_sum(V3 pa, const Bar &b, const int ob[3]) {
         const Dual w = wlw * wh;
         const Dual dx = pa.x - p3.x, dy = pa.y - p3.y, dz = pa.z - p3.z;
         const Dual t = w / dsqrt(dx * dx + dy * dy + dz * dz);
-        if (k == 0) in0 = in0 + t;
-        else if (k == 1) in1 = in1 + t;
-        else if (k == 2) in2 = in2 + t;
-        else in3 = in3 + t;
-        k = (k + 1) & 3;
+        const Dual s = r0 + t;
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        r3 = s;
+        ++k;
       }
     }
   }
+  const int n = k & 3;
+  const Dual in0 = n == 0 ? r0 : n == 1 ? r3 : n == 2 ? r2 : r1;
+  const Dual in1 = n == 0 ? r1 : n == 1 ? r0 : n == 2 ? r3 : r2;
+  const Dual in2 = n == 0 ? r2 : n == 1 ? r1 : n == 2 ? r0 : r3;
+  const Dual in3 = n == 0 ? r3 : n == 1 ? r2 : n == 2 ? r1 : r0;
   return (in0 + in1) + (in2 + in3);
 }
 

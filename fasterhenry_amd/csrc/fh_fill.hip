@@ -310,8 +310,22 @@ __device__ void plan_pair(const fh_filament &fa_in, const fh_filament &fb_in, Pa
 // the same operations as the materialized cloud.
 // neumann.rs:237-266 for one outer point: inner sum over b's cloud in four
 // interleaved lanes, reduced (l0 + l1) + (l2 + l3).
+//
+// The four lane sums rotate through r0..r3 so that the next term always
+// goes to r0 (a select among four accumulators by a lane-varying index costs
+// more than the evaluation's divide); r_j holds lane (k + j) mod 4 after k
+// terms, which lane_total undoes.
+__device__ __forceinline__ double lane_total(double r0, double r1, double r2, double r3, int k) {
+  const int n = k & 3;
+  const double in0 = n == 0 ? r0 : n == 1 ? r3 : n == 2 ? r2 : r1;
+  const double in1 = n == 0 ? r1 : n == 1 ? r0 : n == 2 ? r3 : r2;
+  const double in2 = n == 0 ? r2 : n == 1 ? r1 : n == 2 ? r0 : r3;
+  const double in3 = n == 0 ? r3 : n == 1 ? r2 : n == 2 ? r1 : r0;
+  return (in0 + in1) + (in2 + in3);
+}
+
 __device__ double inner_sum(D3 pa, const Bar &b, const int ob[3]) {
-  double in0 = 0.0, in1 = 0.0, in2 = 0.0, in3 = 0.0;
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
   int k = 0;
   const double s1 = 2.0 * b.half[1], s2 = 2.0 * b.half[2];
   for (int il = 0; il < ob[0]; ++il) {
@@ -330,15 +344,79 @@ __device__ double inner_sum(D3 pa, const Bar &b, const int ob[3]) {
         const double w = wlw * wh;
         const double dx = pa.x - p3.x, dy = pa.y - p3.y, dz = pa.z - p3.z;
         const double t = w / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-        if (k == 0) in0 += t;
-        else if (k == 1) in1 += t;
-        else if (k == 2) in2 += t;
-        else in3 += t;
-        k = (k + 1) & 3;
+        const double s = r0 + t;
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        r3 = s;
+        ++k;
       }
     }
   }
-  return (in0 + in1) + (in2 + in3);
+  return lane_total(r0, r1, r2, r3, k);
+}
+
+// The height nodes of b's cloud for a height order O2 known at compile time
+// (every pair of the point quadrature has O2 = 2 or 3 on the graded
+// cross-sections): the offsets axes[2] * (2 half[2] x_h) and weights w_h
+// the innermost loop adds, formed once per pair with the same operations.
+template <int O2>
+struct HeightNodes {
+  D3 v[O2];
+  double w[O2];
+};
+
+template <int O2>
+__device__ __forceinline__ HeightNodes<O2> height_nodes(const Bar &b) {
+  HeightNodes<O2> h;
+  const double s2 = 2.0 * b.half[2];
+#pragma unroll
+  for (int ih = 0; ih < O2; ++ih) {
+    double xh;
+    rule_on(O2, ih, -0.5, 0.5, xh, h.w[ih]);
+    h.v[ih] = b.axes[2] * (s2 * xh);
+  }
+  return h;
+}
+
+// inner_sum with the height loop unrolled: term ih of a width step goes to
+// lane ih mod 4 of the rotation, which advances by O2 per width step.
+template <int O2>
+__device__ double inner_sum_h(D3 pa, const Bar &b, const int ob[3], const HeightNodes<O2> &h) {
+  double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+  const double s1 = 2.0 * b.half[1];
+  for (int il = 0; il < ob[0]; ++il) {
+    double xl, wl;
+    rule_on(ob[0], il, -b.half[0], b.half[0], xl, wl);
+    const D3 p1 = b.centre + b.axes[0] * xl;
+    for (int iw = 0; iw < ob[1]; ++iw) {
+      double xw, ww;
+      rule_on(ob[1], iw, -0.5, 0.5, xw, ww);
+      const D3 p2 = p1 + b.axes[1] * (s1 * xw);
+      const double wlw = wl * ww;
+      double acc[4] = {r0, r1, r2, r3};
+#pragma unroll
+      for (int ih = 0; ih < O2; ++ih) {
+        const D3 p3 = p2 + h.v[ih];
+        const double w = wlw * h.w[ih];
+        const double dx = pa.x - p3.x, dy = pa.y - p3.y, dz = pa.z - p3.z;
+        acc[ih & 3] += w / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+      }
+      r0 = acc[O2 & 3];
+      r1 = acc[(O2 + 1) & 3];
+      r2 = acc[(O2 + 2) & 3];
+      r3 = acc[(O2 + 3) & 3];
+    }
+  }
+  return lane_total(r0, r1, r2, r3, ob[0] * ob[1] * O2);
+}
+
+// b's cloud sum for one outer point, the height loop unrolled where ob[2] is
+// 2 or 3 (uniform over the wavefront on the deferred path).
+__device__ __forceinline__ double inner_sum_any(D3 pa, const Bar &b, const int ob[3]) {
+  if (ob[2] == 2) return inner_sum_h<2>(pa, b, ob, height_nodes<2>(b));
+  if (ob[2] == 3) return inner_sum_h<3>(pa, b, ob, height_nodes<3>(b));
+  return inner_sum(pa, b, ob);
 }
 
 // Outer cloud point `index` of bar a (il, iw, ih order) and its weight.
@@ -356,7 +434,12 @@ __device__ __forceinline__ void cloud_point(const Bar &a, const int oa[3], int i
   w = wl * ww * wh;
 }
 
-__device__ double point_sum_serial(const Bar &a, const int oa[3], const Bar &b, const int ob[3]) {
+// neumann.rs:237-266: the outer sum over a's cloud in order; O2 > 0 is b's
+// height order, unrolled (0: any order).
+template <int O2>
+__device__ double point_sum_h(const Bar &a, const int oa[3], const Bar &b, const int ob[3]) {
+  HeightNodes<O2 ? O2 : 1> h;
+  if constexpr (O2 > 0) h = height_nodes<O2>(b);
   double total = 0.0;
   const double s1 = 2.0 * a.half[1], s2 = 2.0 * a.half[2];
   for (int il = 0; il < oa[0]; ++il) {
@@ -372,11 +455,22 @@ __device__ double point_sum_serial(const Bar &a, const int oa[3], const Bar &b, 
         double xh, wh;
         rule_on(oa[2], ih, -0.5, 0.5, xh, wh);
         const D3 p3 = p2 + a.axes[2] * (s2 * xh);
-        total += (wlw * wh) * inner_sum(p3, b, ob);
+        double inner;
+        if constexpr (O2 > 0)
+          inner = inner_sum_h<O2>(p3, b, ob, h);
+        else
+          inner = inner_sum(p3, b, ob);
+        total += (wlw * wh) * inner;
       }
     }
   }
   return total;
+}
+
+__device__ double point_sum_serial(const Bar &a, const int oa[3], const Bar &b, const int ob[3]) {
+  if (ob[2] == 2) return point_sum_h<2>(a, oa, b, ob);
+  if (ob[2] == 3) return point_sum_h<3>(a, oa, b, ob);
+  return point_sum_h<0>(a, oa, b, ob);
 }
 
 // ---------------------------------------------------------------------------
@@ -791,6 +885,8 @@ __global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__
   __shared__ double vals[kTile][kTile + 1];
   __shared__ int8_t meth[kTile][kTile + 1];
   __shared__ int8_t done[kTile][kTile + 1];
+  __shared__ unsigned long long order[kTile * kTile];
+  __shared__ int inline_count;
   int ti, tj;
   tile_coords(static_cast<long long>(blockIdx.x), nt, ti, tj);
   const int r = threadIdx.x / kTile, c = threadIdx.x % kTile;
@@ -812,6 +908,9 @@ __global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__
   bool finished = false;
   bool queued = false;
   uint32_t kind = 0;
+  // Sort key of an inline point-quadrature pair: its six orders, so that
+  // pairs with the same cloud shapes share a wavefront below; ~0 otherwise.
+  unsigned long long key = ~0ull;
   if (i < n && j < n && i <= j && !mask_keeps(mask, i, j)) {
     finished = true;  // truncated: exactly 0, never evaluated
     method = kTruncated;
@@ -821,11 +920,11 @@ __global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__
     if (p.orthogonal) {
       finished = true;
     } else if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
-      const double sum = point_sum_serial(p.a, p.oa, p.b, p.ob);
-      value = 1.0e-7 * (sum * p.cos);
       method = kPointQuadrature;
       finished = true;
-      if (!__builtin_isfinite(value)) record_error(err, i, j, n, 2);
+      key = 0;
+      for (int q = 0; q < 3; ++q) key = (key << 5) | static_cast<unsigned>(p.ob[2 - q]);
+      for (int q = 0; q < 3; ++q) key = (key << 5) | static_cast<unsigned>(p.oa[2 - q]);
     } else {
       kind = p.point ? kHeavyPoint : (p.aligned ? kAligned : kSampled);
       queued = true;
@@ -838,7 +937,42 @@ __global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__
   vals[r][c] = value;
   meth[r][c] = method;
   done[r][c] = finished ? 1 : 0;
+  // The inline point quadratures, sorted by their orders (bitonic, one
+  // element per thread) and dealt out in four contiguous quarters, one per
+  // wavefront: lanes of a wavefront then run the same loops instead of the
+  // longest of 64 different ones, and the wavefronts get equal shares.
+  const int t = static_cast<int>(threadIdx.x);
+  order[t] = (key << 8) | static_cast<unsigned>(t);
+  if (t == 0) inline_count = 0;
   __syncthreads();
+  if (key != ~0ull) atomicAdd(&inline_count, 1);
+  for (int k = 2; k <= kTile * kTile; k <<= 1) {
+    for (int h = k >> 1; h > 0; h >>= 1) {
+      const int partner = t ^ h;
+      if (partner > t) {
+        const unsigned long long a = order[t], b = order[partner];
+        if ((a > b) == ((t & k) == 0)) {
+          order[t] = b;
+          order[partner] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  {
+    const int per = (inline_count + 3) / 4, lane = t % 64, pos = (t / 64) * per + lane;
+    const unsigned long long e = lane < per && pos < inline_count ? order[pos] : ~0ull;
+    if ((e >> 8) != (~0ull >> 8)) {
+      const int slot = static_cast<int>(e & 0xff), rr = slot / kTile, cc = slot % kTile;
+      PairPlan p;
+      plan_pair(rows[rr], cols[cc], p);
+      const double v = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
+      if (!__builtin_isfinite(v)) record_error(err, ti * kTile + rr, tj * kTile + cc, n, 2);
+      vals[rr][cc] = v;
+    }
+  }
+  __syncthreads();
+  value = vals[r][c];
   if (finished) {
     L[static_cast<size_t>(i) * n + j] = value;
     if (methods) methods[static_cast<size_t>(i) * n + j] = method;
@@ -870,7 +1004,7 @@ __device__ void deferred_eval(const fh_filament &fa, const fh_filament &fb, uint
       D3 pa;
       double wa;
       cloud_point(p.a, p.oa, idx, pa, wa);
-      lds[idx] = wa * inner_sum(pa, p.b, p.ob);
+      lds[idx] = wa * inner_sum_any(pa, p.b, p.ob);
     }
     __syncthreads();
     double total = 0.0;
