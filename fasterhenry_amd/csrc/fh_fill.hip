@@ -411,14 +411,6 @@ __device__ double inner_sum_h(D3 pa, const Bar &b, const int ob[3], const Height
   return lane_total(r0, r1, r2, r3, ob[0] * ob[1] * O2);
 }
 
-// b's cloud sum for one outer point, the height loop unrolled where ob[2] is
-// 2 or 3 (uniform over the wavefront on the deferred path).
-__device__ __forceinline__ double inner_sum_any(D3 pa, const Bar &b, const int ob[3]) {
-  if (ob[2] == 2) return inner_sum_h<2>(pa, b, ob, height_nodes<2>(b));
-  if (ob[2] == 3) return inner_sum_h<3>(pa, b, ob, height_nodes<3>(b));
-  return inner_sum(pa, b, ob);
-}
-
 // Outer cloud point `index` of bar a (il, iw, ih order) and its weight.
 __device__ __forceinline__ void cloud_point(const Bar &a, const int oa[3], int index, D3 &p,
                                             double &w) {
@@ -682,6 +674,13 @@ __device__ double corner_wave(const Aligned &b, double axial_start, double a, do
 struct Frame {
   double u0, u1, v0, v1, at, acc;
   int depth, stage, axis, pad;
+};
+
+// LDS of a deferred wavefront: the aligned quadrature's stack, or the inner
+// cloud (x, y, z, w) of a heavy point quadrature; one pair uses one of them.
+union DeferredScratch {
+  Frame stack[kStackCapacity];
+  double cloud[4 * kCloudCapacity];
 };
 
 __device__ __forceinline__ double gap_of(double r0, double r1) {
@@ -989,29 +988,55 @@ __global__ void __launch_bounds__(256, 4) fill_tile_kernel(const fh_filament *__
 // One queued pair on one wavefront (the deferred methods): value
 // 1e-7 * integral, its Method, and false for the divergent sampled case.
 __device__ void deferred_eval(const fh_filament &fa, const fh_filament &fb, uint32_t kind,
-                              double *lds, Frame *stack, int lane, double &v, int8_t &method,
-                              bool &ok) {
+                              double *lds, DeferredScratch &sh, int lane, double &v,
+                              int8_t &method, bool &ok) {
   PairPlan p;
   plan_pair(fa, fb, p);
   double integral = 0.0;
   method = kPointQuadrature;
   ok = true;
   if (kind == kHeavyPoint) {
-    // neumann.rs:237-266, lanes over the outer cloud.
-    const int na = p.oa[0] * p.oa[1] * p.oa[2];
+    // neumann.rs:237-266.  b's cloud is formed once into LDS; then four
+    // lanes per outer point i each run one of the four interleaved lane
+    // sums over it (lane s adds the terms j = s, s + 4, ... in order), and
+    // (l0 + l1) + (l2 + l3) is formed across them; the outer sum of
+    // w_i * inner_i runs in order of i.
+    const int na = p.oa[0] * p.oa[1] * p.oa[2], nb = p.ob[0] * p.ob[1] * p.ob[2];
+    double *cx = sh.cloud, *cy = cx + kCloudCapacity, *cz = cy + kCloudCapacity,
+           *cw = cz + kCloudCapacity;
     __syncthreads();
-    for (int idx = lane; idx < na; idx += 64) {
-      D3 pa;
-      double wa;
-      cloud_point(p.a, p.oa, idx, pa, wa);
-      lds[idx] = wa * inner_sum_any(pa, p.b, p.ob);
+    for (int q = lane; q < nb; q += 64) {
+      D3 pb;
+      double wb;
+      cloud_point(p.b, p.ob, q, pb, wb);
+      cx[q] = pb.x;
+      cy[q] = pb.y;
+      cz[q] = pb.z;
+      cw[q] = wb;
+    }
+    __syncthreads();
+    for (int base = 0; base < 4 * na; base += 64) {
+      const int q = base + lane, i = q >> 2, slot = q & 3;
+      double acc = 0.0, wa = 0.0;
+      if (i < na) {
+        D3 pa;
+        cloud_point(p.a, p.oa, i, pa, wa);
+        for (int jj = slot; jj < nb; jj += 4) {
+          const double dx = pa.x - cx[jj], dy = pa.y - cy[jj], dz = pa.z - cz[jj];
+          acc += cw[jj] / __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+        }
+      }
+      // (l0 + l1) + (l2 + l3): each addition is commutative bit for bit
+      const double pair = acc + __shfl_xor(acc, 1);
+      const double inner = pair + __shfl_xor(pair, 2);
+      if (i < na && slot == 0) lds[i] = wa * inner;
     }
     __syncthreads();
     double total = 0.0;
     for (int idx = 0; idx < na; ++idx) total += lds[idx];
     integral = total * p.cos;
   } else if (kind == kAligned) {
-    const double in = aligned_dispatch_wave(p.ab, method, stack, lds);
+    const double in = aligned_dispatch_wave(p.ab, method, sh.stack, lds);
     const double scale = signum(p.cos) / (p.area_a * p.area_b);
     integral = in * scale;
   } else {
@@ -1078,7 +1103,7 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
                                                            unsigned long long *__restrict__ err,
                                                            int skip_aligned) {
   __shared__ double lds[kCloudCapacity];
-  __shared__ Frame stack[kStackCapacity];
+  __shared__ DeferredScratch scratch_lds;
   const unsigned int count = *queue_count;
   const int lane = static_cast<int>(threadIdx.x);
   for (unsigned int e = blockIdx.x; e < count; e += gridDim.x) {
@@ -1089,7 +1114,7 @@ __global__ void __launch_bounds__(64) fill_deferred_kernel(const fh_filament *__
     double v;
     int8_t method;
     bool ok;
-    deferred_eval(F[i], F[j], kind, lds, stack, lane, v, method, ok);
+    deferred_eval(F[i], F[j], kind, lds, scratch_lds, lane, v, method, ok);
     if (lane == 0) {
       if (!ok) {
         record_error(err, i, j, n, 1);
@@ -1144,7 +1169,7 @@ __global__ void __launch_bounds__(64) pairs_deferred_kernel(
     const unsigned int *__restrict__ queue_count, unsigned long long *__restrict__ err,
     int skip_aligned) {
   __shared__ double lds[kCloudCapacity];
-  __shared__ Frame stack[kStackCapacity];
+  __shared__ DeferredScratch scratch_lds;
   const unsigned int count = *queue_count;
   const int lane = static_cast<int>(threadIdx.x);
   for (unsigned int e = blockIdx.x; e < count; e += gridDim.x) {
@@ -1156,7 +1181,7 @@ __global__ void __launch_bounds__(64) pairs_deferred_kernel(
     double v;
     int8_t method;
     bool ok;
-    deferred_eval(F[i], F[j], kind, lds, stack, lane, v, method, ok);
+    deferred_eval(F[i], F[j], kind, lds, scratch_lds, lane, v, method, ok);
     if (lane == 0) {
       if (!ok) {
         record_error(err, i, j, n, 1);
