@@ -28,16 +28,19 @@ sys.path.insert(0, ROOT)
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 
 # HBM bytes per update_left_kernel launch on the default workload (meander(49),
-# 41 frequencies, refined, two parts), mean over the sweep's 228 launches:
-# rocprofv3 FETCH_SIZE 885.6 MB + WRITE_SIZE 65.2 MB, each scaled by its
-# reading for a plain 512 MiB device copy (bench.py --copy-mib 512).
-UPDATE_TRAFFIC_BYTES = 885.6e6 + 65.2e6
+# 41 frequencies, refined, outer block 256), per frequency in the launch:
+# the mean over a three-part sweep's 342 launches (41/3 frequencies each) of
+# rocprofv3 FETCH_SIZE 590.5 MB + WRITE_SIZE 43.5 MB, each scaled by its
+# reading for a plain 512 MiB device copy (bench.py --copy-mib 512: the copy
+# kernel reads 262 155 KiB, factor 2.0; writes 524 288 KiB, factor 1.0).
+# A launch of k frequencies moves k times this.
+UPDATE_TRAFFIC_BYTES_PER_FREQUENCY = (590.5e6 + 43.5e6) / (41 / 3)
 
 # HBM bytes per pFFT operator product (V = 2) on meander(157): the sum over
 # the product's kernels (spread, line passes, column DFTs, gather, near
-# field) of rocprofv3 FETCH_SIZE (4 630 MB) and WRITE_SIZE (2 359 MB) per
+# field) of rocprofv3 FETCH_SIZE (3 860 MB) and WRITE_SIZE (2 366 MB) per
 # product, scaled as above.
-PFFT_TRAFFIC_BYTES = 4630.5e6 + 2358.9e6
+PFFT_TRAFFIC_BYTES = 3860.1e6 + 2366.1e6
 
 
 def cpu_model() -> str:
@@ -541,10 +544,10 @@ def main() -> int:
                      "frac": achieved / peak,
                      "flops": "algorithmic: 8 real flops per complex multiply-add of the update "
                               "(FP32-equivalent), (4/3) m^3 per frequency over all launches",
-                     "traffic": UPDATE_TRAFFIC_BYTES if (args.bars, args.freqs, args.precision,
-                                                         system.schedule()["sweep_parts"],
-                                                         system.schedule()["outer_block"]) ==
-                     (49, 41, "fp32_refined", 2, 256) else None,
+                     "traffic": UPDATE_TRAFFIC_BYTES_PER_FREQUENCY * args.freqs /
+                     system.schedule()["sweep_parts"] if (args.bars, args.freqs, args.precision,
+                                                           system.schedule()["outer_block"]) ==
+                     (49, 41, "fp32_refined", 256) else None,
                      "kernel": "outer trailing update of the L D L^T factorization "
                                f"({peak_kind} matrix cores)",
                      "matrix_core_tflops": unit_rate,

@@ -598,11 +598,17 @@ __global__ void pack_sources_kernel(const double *__restrict__ dirs, const doubl
 
 // Sources on the compact grid: q_s(g) = sum over the transposed projection
 // (filaments in order) of (w l_c) x_v, in FP64, stored as channel values.
+// Workgroup b runs on XCD b mod 8; it takes the block of grid points
+// (b mod 8) * per + b / 8 (the grid is a multiple of 8 workgroups), so each
+// XCD spreads one contiguous eighth of the grid and the packed records of
+// its filaments stay in that XCD's L2 while the CSR streams through.
 template <typename T2>
 __global__ void __launch_bounds__(256) spread_compact_kernel(
     T2 *__restrict__ grid, const int64_t *__restrict__ soff, const uint32_t *__restrict__ sfil,
     const double *__restrict__ sw, const double2 *__restrict__ pk, int64_t G, SourceSet ss) {
-  const int64_t g = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int64_t per = gridDim.x / 8;
+  const int64_t block = static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int64_t g = block * blockDim.x + threadIdx.x;
   if (g >= G) return;
   double acc[kBatchSources];
 #pragma unroll
@@ -625,14 +631,18 @@ __global__ void __launch_bounds__(256) spread_compact_kernel(
 // far_v[i] += l_c(i) sum_e W_e phi_s(g_e) for the batch's sources (the
 // components of one vector in ascending order): one wavefront per filament,
 // lanes striding its projection entries, partial sums combined in a fixed
-// butterfly order.
+// butterfly order.  Filaments are dealt to XCDs in contiguous eighths (as
+// in the spread), so neighbouring filaments read their grid points from
+// one L2.
 template <typename T2>
 __global__ void __launch_bounds__(256) gather_compact_kernel(
     const T2 *__restrict__ grid, const int64_t *__restrict__ poff, const uint32_t *__restrict__ pidx,
     const double *__restrict__ pw, const double *__restrict__ dirs, int64_t n, int64_t G, SourceSet ss,
     double *__restrict__ far) {
   const int lane = threadIdx.x & 63;
-  const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t per = gridDim.x / 8;
+  const int64_t block = static_cast<int64_t>(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int64_t i = block * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
   double sum[kBatchSources];
 #pragma unroll
@@ -663,6 +673,9 @@ __global__ void __launch_bounds__(256) gather_compact_kernel(
 unsigned blocks_for(int64_t total) {
   return static_cast<unsigned>(std::min<int64_t>((total + 255) / 256, 1 << 20));
 }
+// Workgroups of 256 covering `total` threads, rounded up to a multiple of 8
+// (one share per XCD; the kernel maps them back to contiguous blocks).
+unsigned xcd_blocks_for(int64_t total) { return static_cast<unsigned>(((total + 255) / 256 + 7) / 8 * 8); }
 
 // Host thread pool helper: f(begin, end) over [0, n).
 template <typename F>
@@ -1441,20 +1454,20 @@ fh_status PfftOperator::apply(const double *d_x, double *d_y, int V, hipStream_t
       if (op.conv.single()) {
         float2 *grid = static_cast<float2 *>(op.conv.grid());
         pack_sources_kernel<<<blocks_for(n), 256, 0, stream>>>(op.dirs.ptr, d_x, n, ss, op.packed.ptr);
-        spread_compact_kernel<float2><<<blocks_for(G), 256, 0, stream>>>(
+        spread_compact_kernel<float2><<<xcd_blocks_for(G), 256, 0, stream>>>(
             grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.packed.ptr, G, ss);
         FH_HIP_TRY(hipGetLastError());
         if ((st = op.conv.convolve(C, stream))) return st;
-        gather_compact_kernel<float2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(
+        gather_compact_kernel<float2><<<xcd_blocks_for(64 * n), 256, 0, stream>>>(
             grid, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, op.dirs.ptr, n, G, ss, op.far.ptr);
       } else {
         double2 *grid = static_cast<double2 *>(op.conv.grid());
         pack_sources_kernel<<<blocks_for(n), 256, 0, stream>>>(op.dirs.ptr, d_x, n, ss, op.packed.ptr);
-        spread_compact_kernel<double2><<<blocks_for(G), 256, 0, stream>>>(
+        spread_compact_kernel<double2><<<xcd_blocks_for(G), 256, 0, stream>>>(
             grid, op.spread_off.ptr, op.spread_fil.ptr, op.spread_w.ptr, op.packed.ptr, G, ss);
         FH_HIP_TRY(hipGetLastError());
         if ((st = op.conv.convolve(C, stream))) return st;
-        gather_compact_kernel<double2><<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(
+        gather_compact_kernel<double2><<<xcd_blocks_for(64 * n), 256, 0, stream>>>(
             grid, op.proj_off.ptr, op.proj_idx.ptr, op.proj_w.ptr, op.dirs.ptr, n, G, ss, op.far.ptr);
       }
       FH_HIP_TRY(hipGetLastError());
