@@ -171,7 +171,7 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // outside the block) and D^-1 for block blk of frequency f.
 //
 // One workgroup per frequency; lane r of every wave holds row r, and wave g
-// of the eight holds the columns j = g (mod 8) in registers (8 slots), so a
+// of the eight holds the columns 8g .. 8g + 7 in registers (8 slots), so a
 // column step costs each lane 8 complex multiply-adds and one barrier, with
 // two waves per SIMD to cover each other's dependency stalls.  Pivot columns
 // reach the other waves through a two-entry LDS ring.  Linv is built in the
@@ -182,6 +182,15 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // A[r][j] -= l_r A[j][c] with A[j][c] from the ring; for j < c,
 // X[r][j] -= l_r X[c][j] with X[c][j] read from lane c; for j = c the slot
 // becomes X[r][c] = delta_rc - l_r.
+//
+// Column c = 8q + t is held by wave q, slot t.  The step loop is unrolled
+// over t, so in every wave each slot's role at a step is fixed by one
+// wave-uniform branch: waves below q hold only columns left of c (all X
+// updates), waves above q only columns right of it (all A updates), and in
+// wave q the roles split at slot t, known at compile time.  (A round-robin
+// column assignment makes every slot's role a run-time choice, and the
+// selects between both candidate updates tripled the instructions of a
+// step.)
 //
 // The working type C is the factor's type: FP64 for the FP64 factorization,
 // FP32 for the FP32 one (whose products elsewhere carry split-bf16 rounding
@@ -221,6 +230,22 @@ __device__ __forceinline__ float readlane_c(float v, int lane) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
+// Slot update of the elimination step at column c: X role (column left of
+// c: X[r][j] -= l X[c][j], X[c][j] from lane c) or A role (column right of
+// c: A[r][j] -= l A[j][c], A[j][c] from the ring).
+template <typename C>
+__device__ __forceinline__ void diag_x(C &mr, C &mi, Zc<C> l, int c) {
+  const Zc<C> pr = zmulc(l, Zc<C>{readlane_c(mr, c), readlane_c(mi, c)});
+  mr = mr - pr.re;
+  mi = mi - pr.im;
+}
+template <typename C>
+__device__ __forceinline__ void diag_a(C &mr, C &mi, Zc<C> l, C br, C bi) {
+  const Zc<C> pr = zmulc(l, Zc<C>{br, bi});
+  mr = mr - pr.re;
+  mi = mi - pr.im;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ re, T *__restrict__ im,
                                                    int64_t ld, int64_t mat_stride, int k0, int w,
@@ -229,6 +254,7 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
                                                    T *__restrict__ dinv_im,
                                                    int *__restrict__ singular) {
   using C = T;
+  constexpr int S = kDiagSlots;
   const int f = blockIdx.x;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -238,10 +264,10 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
   // L11 by columns, D and D^-1: written to memory after the loop, so that
   // no global store is outstanding at the barriers of the column steps
   __shared__ C lsr[kB][kB], lsi[kB][kB], dsr[kB], dsi[kB], vsr[kB], vsi[kB];
-  C mr[kDiagSlots], mi[kDiagSlots];
+  C mr[S], mi[S];
 #pragma unroll
-  for (int js = 0; js < kDiagSlots; ++js) {
-    const int j = kDiagWaves * js + g;
+  for (int js = 0; js < S; ++js) {
+    const int j = S * g + js;
     const bool in = r < w && j <= r;
     const int64_t a = static_cast<int64_t>(k0 + min(j, rc)) * ld + k0 + rc;
     const C vr = Re[a], vi = Im[a];
@@ -254,53 +280,74 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
   }
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
   __syncthreads();
-  for (int c = 0; c < w; ++c) {
-    const int b = c & 1;
-    const Zc<C> d{cbr[b][c], cbi[b][c]};
-    const Zc<C> inv = zrecipc(d);
-    const bool below = r > c && r < w;
-    const Zc<C> lv = zmulc(Zc<C>{cbr[b][r], cbi[b][r]}, inv);
-    const Zc<C> l{below ? lv.re : C(0), below ? lv.im : C(0)};
-    if (g == 0) {
-      lsr[c][r] = l.re;
-      lsi[c][r] = l.im;
-      if (r == 0) {
-        dsr[c] = d.re;
-        dsi[c] = d.im;
-        vsr[c] = inv.re;
-        vsi[c] = inv.im;
-      }
-    }
-    // Written without branches on the slot's state (selects): a branch here
-    // makes the compiler copy the register arrays through every arm.
-    // All reads of the ring entry before its other entry is written.
-    C bcr[kDiagSlots], bci[kDiagSlots];
+  for (int q = 0; q < kDiagWaves; ++q) {
 #pragma unroll
-    for (int js = 0; js < kDiagSlots; ++js) {
-      bcr[js] = cbr[b][kDiagWaves * js + g];
-      bci[js] = cbi[b][kDiagWaves * js + g];
-    }
-    const C delta = r == c ? C(1) : C(0);
-#pragma unroll
-    for (int js = 0; js < kDiagSlots; ++js) {
-      const int j = kDiagWaves * js + g;
-      const bool from_ring = j > c;
-      const C xcr = readlane_c(mr[js], c), xci = readlane_c(mi[js], c);
-      const Zc<C> pr = zmulc(l, Zc<C>{from_ring ? bcr[js] : xcr, from_ring ? bci[js] : xci});
-      const C ur = mr[js] - pr.re, ui = mi[js] - pr.im;
-      mr[js] = j == c ? delta - l.re : ur;
-      mi[js] = j == c ? C(0) - l.im : ui;
-      if (j == c + 1) {  // column c + 1 goes to the other ring entry
-        cbr[b ^ 1][r] = mr[js];
-        cbi[b ^ 1][r] = mi[js];
+    for (int t = 0; t < S; ++t) {
+      const int c = S * q + t;
+      if (c >= w) break;
+      const int b = c & 1;
+      const Zc<C> d{cbr[b][c], cbi[b][c]};
+      const Zc<C> inv = zrecipc(d);
+      const bool below = r > c && r < w;
+      const Zc<C> lv = zmulc(Zc<C>{cbr[b][r], cbi[b][r]}, inv);
+      const Zc<C> l{below ? lv.re : C(0), below ? lv.im : C(0)};
+      if (g == 0) {
+        lsr[c][r] = l.re;
+        lsi[c][r] = l.im;
+        if (r == 0) {
+          dsr[c] = d.re;
+          dsi[c] = d.im;
+          vsr[c] = inv.re;
+          vsi[c] = inv.im;
+        }
       }
+      if (g < q) {
+#pragma unroll
+        for (int js = 0; js < S; ++js) diag_x(mr[js], mi[js], l, c);
+      } else if (g > q) {
+        // all reads of the ring entry before its other entry is written
+        C bcr[S], bci[S];
+#pragma unroll
+        for (int js = 0; js < S; ++js) {
+          bcr[js] = cbr[b][S * g + js];
+          bci[js] = cbi[b][S * g + js];
+        }
+#pragma unroll
+        for (int js = 0; js < S; ++js) diag_a(mr[js], mi[js], l, bcr[js], bci[js]);
+        if (t == S - 1 && g == q + 1) {  // column c + 1 = 8(q + 1): slot 0 here
+          cbr[b ^ 1][r] = mr[0];
+          cbi[b ^ 1][r] = mi[0];
+        }
+      } else {
+        C bcr[S], bci[S];
+#pragma unroll
+        for (int js = t + 1; js < S; ++js) {
+          bcr[js] = cbr[b][S * g + js];
+          bci[js] = cbi[b][S * g + js];
+        }
+#pragma unroll
+        for (int js = 0; js < S; ++js) {
+          if (js < t) {
+            diag_x(mr[js], mi[js], l, c);
+          } else if (js == t) {
+            mr[js] = (r == c ? C(1) : C(0)) - l.re;
+            mi[js] = C(0) - l.im;
+          } else {
+            diag_a(mr[js], mi[js], l, bcr[js], bci[js]);
+          }
+        }
+        if (t + 1 < S) {  // column c + 1 is this wave's next slot
+          cbr[b ^ 1][r] = mr[t + 1];
+          cbi[b ^ 1][r] = mi[t + 1];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
 #pragma unroll
-  for (int js = 0; js < kDiagSlots; ++js) {
-    const int j = kDiagWaves * js + g;
+  for (int js = 0; js < S; ++js) {
+    const int j = S * g + js;
     const bool in = r < w && j < w;
     linvt_re[lbase + j * kB + r] = in ? mr[js] : T(0);
     linvt_im[lbase + j * kB + r] = in ? mi[js] : T(0);
