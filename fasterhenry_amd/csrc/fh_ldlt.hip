@@ -126,6 +126,91 @@ __host__ __device__ __forceinline__ int64_t split_block(int kc, int g, int G) {
   return static_cast<int64_t>(kc) * G - static_cast<int64_t>(kc) * (kc - 1) + (g - 2 * kc);
 }
 
+// Left-looking update of one strip, done by one wave inside the kernel that
+// next reads it: A[r][s] -= sum over the chunks kc in [kc_beg, kc_end) of
+// W[r][k] L[s][k], for the 64 rows [rb, rb + 64) (r < rend) and the 16
+// columns [cb, cb + 16) (s < cend, s <= r), from the split copies.  The
+// products, their order and the final subtraction are those of
+// update_left_kernel for the same entries, so the bits are the same.
+__device__ __forceinline__ void left_strip_update(float *__restrict__ Re, float *__restrict__ Im,
+                                                  int64_t ld, int rb, int rend, int cb, int cend,
+                                                  int kc_beg, int kc_end, const bf16x8 *lsplit,
+                                                  const bf16x8 *wsplit, int64_t split_stride,
+                                                  int G, int lane) {
+  const __amdgpu_buffer_rsrc_t rL = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<bf16x8 *>(lsplit), 0, static_cast<int>(split_stride * 16), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rW = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<bf16x8 *>(wsplit), 0, static_cast<int>(split_stride * 16), 0x00020000);
+  const int voff = lane * 16;
+  const int gs = min(cb / 16, G - 1);
+  int gr[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) gr[b] = min(rb / 16 + b, G - 1);
+  f4 accr[4], acci[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    accr[b] = f4{0, 0, 0, 0};
+    acci[b] = f4{0, 0, 0, 0};
+  }
+  for (int kc = kc_beg; kc < kc_end; ++kc) {
+    bf16x8 p[4], q[4][4];
+    const int so = static_cast<int>(split_block(kc, gs, G) * 4096);
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl)
+      p[pl] = __builtin_bit_cast(bf16x8,
+                                 __builtin_amdgcn_raw_buffer_load_b128(rL, voff + 1024 * pl, so, 0));
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int sq = static_cast<int>(split_block(kc, gr[b], G) * 4096);
+#pragma unroll
+      for (int pl = 0; pl < 4; ++pl)
+        q[b][pl] = __builtin_bit_cast(
+            bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rW, voff + 1024 * pl, sq, 0));
+    }
+    const bf16x8 nih = neg8(p[2]), nil = neg8(p[3]);
+#pragma unroll
+    for (int jb = 0; jb < 4; ++jb) {
+      f4 cr = accr[jb], ci = acci[jb];
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], q[jb][0], cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], q[jb][1], cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[1], q[jb][0], cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][2], cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][3], cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, q[jb][2], cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], q[jb][2], ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[0], q[jb][3], ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[1], q[jb][2], ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[2], q[jb][0], ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[2], q[jb][1], ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[3], q[jb][0], ci, 0, 0, 0);
+      accr[jb] = cr;
+      acci[jb] = ci;
+    }
+  }
+  const int kq = lane >> 4, x16 = lane & 15;
+  float cr[4][4], ci[4][4];
+#pragma unroll
+  for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t s = min(cb + 4 * kq + v, cend - 1);
+      const int64_t r = min(rb + 16 * jb + x16, rend - 1);
+      cr[jb][v] = Re[s * ld + r];
+      ci[jb][v] = Im[s * ld + r];
+    }
+#pragma unroll
+  for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t s = cb + 4 * kq + v;
+      const int64_t r = rb + 16 * jb + x16;
+      if (s <= r && r < rend && s < cend) {
+        Re[s * ld + r] = cr[jb][v] - accr[jb][v];
+        Im[s * ld + r] = ci[jb][v] - acci[jb][v];
+      }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forming Z(w) = R + j w L on the lower triangle (solve.rs:1003-1009), all
 // frequencies of the batch per thread: l_mesh is read once.  l_mesh is
@@ -252,7 +337,11 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
                                                    int blk, int nblk, T *__restrict__ linvt_re,
                                                    T *__restrict__ linvt_im, T *__restrict__ dinv_re,
                                                    T *__restrict__ dinv_im,
-                                                   int *__restrict__ singular) {
+                                                   int *__restrict__ singular,
+                                                   const bf16x8 *__restrict__ lsplit,
+                                                   const bf16x8 *__restrict__ wsplit,
+                                                   int64_t split_stride, int G, int kc_beg,
+                                                   int kc_end) {
   using C = T;
   constexpr int S = kDiagSlots;
   const int f = blockIdx.x;
@@ -260,6 +349,17 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = threadIdx.x & 63;
   const int rc = min(r, w - 1);
+  if constexpr (std::is_same<T, float>::value) {
+    // left-looking path: the block's own update by the columns of its outer
+    // block to its left (chunks [kc_beg, kc_end)), one 16-column strip per
+    // wave of the first four, before the block is read
+    if (kc_end > kc_beg) {
+      if (g < 4)
+        left_strip_update(Re, Im, ld, k0, k0 + w, k0 + 16 * g, k0 + w, kc_beg, kc_end,
+                          lsplit + f * split_stride, wsplit + f * split_stride, split_stride, G, r);
+      __syncthreads();
+    }
+  }
   __shared__ C cbr[2][kB], cbi[2][kB];
   // L11 by columns, D and D^-1: written to memory after the loop, so that
   // no global store is outstanding at the barriers of the column steps
@@ -434,7 +534,10 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
                                                     const T *__restrict__ dinv_im,
                                                     bf16x8 *__restrict__ lsplit,
                                                     bf16x8 *__restrict__ wsplit,
-                                                    int64_t split_stride, int G, int write_w) {
+                                                    int64_t split_stride, int G, int write_w,
+                                                    const bf16x8 *__restrict__ lsrc,
+                                                    const bf16x8 *__restrict__ wsrc, int kc_beg,
+                                                    int kc_end) {
   const int f = blockIdx.y;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   T *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
@@ -442,6 +545,18 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
   const int r0 = k0 + w + blockIdx.x * kT;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if constexpr (std::is_same<T, float>::value) {
+    // left-looking path: these panel rows' update by the columns of the
+    // outer block left of this block (chunks [kc_beg, kc_end)), one
+    // 16-column strip per wave, before the rows are read; the split copies
+    // of earlier blocks, not the ones this launch writes
+    if (kc_end > kc_beg) {
+      left_strip_update(Re, Im, ld, r0, N, k0 + 16 * __builtin_amdgcn_readfirstlane(wave), k0 + w,
+                        kc_beg, kc_end, lsrc + f * split_stride, wsrc + f * split_stride,
+                        split_stride, G, lane);
+      __syncthreads();
+    }
+  }
   const int wi = 32 * (wave >> 1), wj = 32 * (wave & 1);
   const int kq = lane >> 4, x16 = lane & 15;
   // Operands straight from memory into the matrix-core layout (lane l:
@@ -1603,17 +1718,14 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
     for (int k0 = K0; k0 < KE; k0 += kB) {
       const int w = std::min(kB, KE - k0);
       const int blk = k0 / kB;
-      if constexpr (std::is_same<T, float>::value) {
-        // left-looking inside the outer block too: the earlier 64-column
-        // blocks of this outer block, from their split copies
-        if (sb && k0 > K0) {
-          const fh_status st =
-              launch_left_update(L, F, *sb, K, k0, w, K0 / 32, k0 / 32, stream, prof);
-          if (st) return st;
-        }
-      }
+      // left-looking inside the outer block too: the earlier 64-column
+      // blocks of this outer block, from their split copies, applied by the
+      // diagonal-block and panel kernels to their own rows
+      const int ikc_beg = sb && k0 > K0 ? K0 / 32 : 0, ikc_end = sb && k0 > K0 ? k0 / 32 : 0;
       diag_kernel<T><<<K, 64 * kDiagWaves, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
-                                            F.lt_re, F.lt_im, F.di_re, F.di_im, d_singular);
+                                            F.lt_re, F.lt_im, F.di_re, F.di_im, d_singular,
+                                            sb ? sb->l : nullptr, sb ? sb->w : nullptr,
+                                            sb ? sb->stride : 0, sb ? sb->G : 0, ikc_beg, ikc_end);
       FH_HIP_TRY(hipGetLastError());
       const int rows = L.N - (k0 + w);
       if (rows <= 0) continue;
@@ -1624,7 +1736,8 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
       panel_kernel<T><<<dim3(ptiles, K), 256, 0, stream>>>(
           F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk, L.N, F.wre, F.wim, L.w_stride,
           k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im, emit ? sb->l : nullptr,
-          emit ? sb->w : nullptr, sb ? sb->stride : 0, sb ? sb->G : 0, sb ? 0 : 1);
+          emit ? sb->w : nullptr, sb ? sb->stride : 0, sb ? sb->G : 0, sb ? 0 : 1,
+          sb ? sb->l : nullptr, sb ? sb->w : nullptr, ikc_beg, ikc_end);
       FH_HIP_TRY(hipGetLastError());
       if (sb) continue;
       if (k0 + w < KE)
