@@ -639,6 +639,17 @@ struct Frame {
   int depth, stage, axis, pad;
 };
 
+// LDS of a deferred wavefront: the aligned quadrature's stack, or the inner
+// cloud (x, y, z, w) of a heavy point quadrature; one pair uses one of them.
+union DeferredScratch {
+  Frame stack[kStackCapacity];
+  Dual cloud[4 * kCloudCapacity];
+};
+
+__device__ __forceinline__ Dual shfl_xor_dual(Dual a, int m) {
+  return {__shfl_xor(a.v, m), __shfl_xor(a.d, m)};
+}
+
 __device__ __forceinline__ double gap_of(double r0, double r1) {
   if (r0 > 0.0) return r0;
   if (r1 < 0.0) return -r1;
@@ -776,24 +787,48 @@ __device__ Dual aligned_dispatch_wave(const Aligned &b, Frame *stack, Dual *lds)
 }
 
 // The fill's deferred evaluation (one wave), on duals: 1e-7 x integral.
-__device__ __forceinline__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual *lds, Frame *stack, int lane, bool &ok) {
+__device__ __forceinline__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual *lds, DeferredScratch &sh,
+                                              int lane, bool &ok) {
   Dual integral = cst(0.0);
   ok = true;
   if (kind == kHeavyPoint) {
-    const int na = p.oa[0] * p.oa[1] * p.oa[2];
+    // As the fill's: b's cloud formed once in LDS, four lanes per outer
+    // point i each running one of the four interleaved lane sums, combined
+    // as (l0 + l1) + (l2 + l3); the outer sum in order of i.
+    const int na = p.oa[0] * p.oa[1] * p.oa[2], nb = p.ob[0] * p.ob[1] * p.ob[2];
+    Dual *cx = sh.cloud, *cy = cx + kCloudCapacity, *cz = cy + kCloudCapacity, *cw = cz + kCloudCapacity;
     __syncthreads();
-    for (int idx = lane; idx < na; idx += 64) {
-      V3 pa;
-      Dual wa;
-      cloud_point(p.a, p.oa, idx, pa, wa);
-      lds[idx] = wa * inner_sum(pa, p.b, p.ob);
+    for (int q = lane; q < nb; q += 64) {
+      V3 pb;
+      Dual wb;
+      cloud_point(p.b, p.ob, q, pb, wb);
+      cx[q] = pb.x;
+      cy[q] = pb.y;
+      cz[q] = pb.z;
+      cw[q] = wb;
+    }
+    __syncthreads();
+    for (int base = 0; base < 4 * na; base += 64) {
+      const int q = base + lane, i = q >> 2, slot = q & 3;
+      Dual acc = cst(0.0), wa = cst(0.0);
+      if (i < na) {
+        V3 pa;
+        cloud_point(p.a, p.oa, i, pa, wa);
+        for (int jj = slot; jj < nb; jj += 4) {
+          const Dual dx = pa.x - cx[jj], dy = pa.y - cy[jj], dz = pa.z - cz[jj];
+          acc = acc + cw[jj] / dsqrt(dx * dx + dy * dy + dz * dz);
+        }
+      }
+      const Dual pair = acc + shfl_xor_dual(acc, 1);
+      const Dual inner = pair + shfl_xor_dual(pair, 2);
+      if (i < na && slot == 0) lds[i] = wa * inner;
     }
     __syncthreads();
     Dual total = cst(0.0);
     for (int idx = 0; idx < na; ++idx) total = total + lds[idx];
     integral = total * p.cos;
   } else if (kind == kAlignedKind) {
-    const Dual in = aligned_dispatch_wave(p.ab, stack, lds);
+    const Dual in = aligned_dispatch_wave(p.ab, sh.stack, lds);
     const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
     integral = in * scale;
   } else {
@@ -985,7 +1020,7 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
                                                        const unsigned *__restrict__ sidx, unsigned count, int P,
                                                        double *__restrict__ dl, int *__restrict__ bad) {
   __shared__ Dual lds[kCloudCapacity];
-  __shared__ Frame stack[kStackCapacity];
+  __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
   const size_t total = static_cast<size_t>(count) * P;
   for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
@@ -1000,7 +1035,7 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
       Plan p;
       plan_pair(F[f], T[qn + f], F[g], T[qn + g], p);
       bool ok;
-      d = deferred_eval(p, kind, lds, stack, lane, ok).d;
+      d = deferred_eval(p, kind, lds, scratch_lds, lane, ok).d;
       if (lane == 0 && !ok) atomicOr(bad, 1);
       if (lane == 0 && p.unsupported) atomicOr(bad, 2);
     }
@@ -1065,7 +1100,7 @@ __global__ void __launch_bounds__(64) dpair_kernel(const fh_filament *__restrict
                                                    const uint4 *__restrict__ pairs, size_t count,
                                                    double *__restrict__ value, double *__restrict__ deriv) {
   __shared__ Dual lds[kCloudCapacity];
-  __shared__ Frame stack[kStackCapacity];
+  __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
   for (size_t e = blockIdx.x; e < count; e += gridDim.x) {
     const uint4 pr = pairs[e];
@@ -1080,7 +1115,7 @@ __global__ void __launch_bounds__(64) dpair_kernel(const fh_filament *__restrict
         v = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
       } else {
         const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled);
-        v = deferred_eval(p, kind, lds, stack, lane, ok);
+        v = deferred_eval(p, kind, lds, scratch_lds, lane, ok);
       }
     }
     if (lane == 0) {
