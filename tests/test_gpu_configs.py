@@ -178,9 +178,21 @@ def test_full_batch_graded_members_match_the_oracle():
     order = np.argsort(counts)
     pick = [int(order[0]), int(order[1]), int(order[len(order) // 3])]
     zs64, _ = solve_batch(probs, BATCH_FREQS, precision="fp64", concurrency=4)
-    zsr, _ = solve_batch(probs, BATCH_FREQS, precision="fp32_refined", concurrency=4)
+    zsr, infos = solve_batch(probs, BATCH_FREQS, precision="fp32_refined", concurrency=4)
     for k in range(len(probs)):
         err = max(rel_frobenius(zsr[k][j], zs64[k][j]) for j in range(len(BATCH_FREQS)))
+        if err > 1e-10:
+            # name the side that moved: each against its own single solve
+            packed = model.pack(*probs[k])
+            system = MeshSystem.assemble_packed(packed, probs[k][1], "fp64")
+            z1, _ = system.sweep_raw(BATCH_FREQS)
+            system.set_precision("fp32_refined")
+            zr1, _ = system.sweep_raw(BATCH_FREQS)
+            system.close()
+            e64 = max(rel_frobenius(zs64[k][j], z1[j]) for j in range(len(BATCH_FREQS)))
+            er = max(rel_frobenius(zsr[k][j], zr1[j]) for j in range(len(BATCH_FREQS)))
+            print(f"problem {k}: batch fp64 vs single {e64:.2e}, batch refined vs single {er:.2e}, "
+                  f"info {infos[k]}", flush=True)
         assert err <= 1e-10, (k, err)
     sub = [0, 5, 10, 15]
     for k in pick:
