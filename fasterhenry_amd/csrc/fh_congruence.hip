@@ -70,8 +70,16 @@ __global__ void __launch_bounds__(256) short_short_kernel(
     out[static_cast<int64_t>(row0 + jj) * loops + (row0 + ii)] = vals[c][r];
 }
 
+// Entries of a long row taken per batch: their loads are issued together,
+// then added in row order, so a thread has kBatch (or 2 kBatch) loads in
+// flight instead of one (the sums are latency-bound: a few thousand threads
+// each walk a row of thousands of entries).
+constexpr int kBatch = 16;
+
 // out[i][j] and out[j][i] for a long row j and the short rows i < j listed
-// in short_rows.
+// in short_rows: sum over e in row j (in order) of s_e (sum over f in row i
+// of s_f L[b_e][b_f]).  Rows i of two entries (bundle loops) keep both
+// columns in registers; longer short rows take the general loop.
 __global__ void short_long_kernel(const double *__restrict__ L, int64_t n,
                                   const int64_t *__restrict__ row_ptr,
                                   const int32_t *__restrict__ branch,
@@ -83,12 +91,38 @@ __global__ void short_long_kernel(const double *__restrict__ L, int64_t n,
   const int i = short_rows[t];
   if (i >= j) return;
   const int64_t ib = row_ptr[i], ie = row_ptr[i + 1];
+  const int64_t jb = row_ptr[j], je = row_ptr[j + 1];
   double acc = 0.0;
-  for (int64_t e = row_ptr[j]; e < row_ptr[j + 1]; ++e) {
-    const int64_t b = branch[e];
-    double prod = 0.0;
-    for (int64_t f = ib; f < ie; ++f) prod += sign[f] * L[b * n + branch[f]];
-    acc += sign[e] * prod;
+  if (ie - ib == 2) {
+    const int64_t c0 = branch[ib], c1 = branch[ib + 1];
+    const double s0 = sign[ib], s1 = sign[ib + 1];
+    for (int64_t e0 = jb; e0 < je; e0 += kBatch) {
+      double v0[kBatch], v1[kBatch], se[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int64_t e = e0 + u < je ? e0 + u : jb;
+        const int64_t b = branch[e];
+        se[u] = sign[e];
+        v0[u] = L[b * n + c0];
+        v1[u] = L[b * n + c1];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        if (e0 + u < je) {
+          double prod = 0.0;
+          prod += s0 * v0[u];
+          prod += s1 * v1[u];
+          acc += se[u] * prod;
+        }
+      }
+    }
+  } else {
+    for (int64_t e = jb; e < je; ++e) {
+      const int64_t b = branch[e];
+      double prod = 0.0;
+      for (int64_t f = ib; f < ie; ++f) prod += sign[f] * L[b * n + branch[f]];
+      acc += sign[e] * prod;
+    }
   }
   out[static_cast<int64_t>(i) * loops + j] = acc;
   out[static_cast<int64_t>(j) * loops + i] = acc;
@@ -101,9 +135,20 @@ __global__ void product_kernel(const double *__restrict__ L, int64_t n,
                                const double *__restrict__ sign, int i, double *__restrict__ pv) {
   const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (k >= n) return;
+  const int64_t ib = row_ptr[i], ie = row_ptr[i + 1];
   double acc = 0.0;
-  for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
-    acc += sign[e] * L[static_cast<int64_t>(branch[e]) * n + k];
+  for (int64_t e0 = ib; e0 < ie; e0 += 2 * kBatch) {
+    double v[2 * kBatch], se[2 * kBatch];
+#pragma unroll
+    for (int u = 0; u < 2 * kBatch; ++u) {
+      const int64_t e = e0 + u < ie ? e0 + u : ib;
+      se[u] = sign[e];
+      v[u] = L[static_cast<int64_t>(branch[e]) * n + k];
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * kBatch; ++u)
+      if (e0 + u < ie) acc += se[u] * v[u];
+  }
   pv[k] = acc;
 }
 
@@ -186,7 +231,7 @@ fh_status congruence_inductance(const LoopBasis &basis, size_t n, const double *
         int n_before = 0;
         while (n_before < static_cast<int>(short_rows.size()) && short_rows[n_before] < runs[x].first)
           ++n_before;
-        short_long_kernel<<<(n_before + 255) / 256, 256, 0, stream>>>(
+        short_long_kernel<<<(n_before + 63) / 64, 64, 0, stream>>>(
             d_L, static_cast<int64_t>(n), d_row_ptr.ptr, d_branch.ptr, d_sign.ptr, d_short.ptr,
             n_before, j, d_lmesh, loops);
         FH_HIP_TRY(hipGetLastError());
@@ -195,12 +240,12 @@ fh_status congruence_inductance(const LoopBasis &basis, size_t n, const double *
   for (int32_t j : long_rows) {
     if (!short_rows.empty()) {
       const int ns = static_cast<int>(short_rows.size());
-      short_long_kernel<<<(ns + 255) / 256, 256, 0, stream>>>(
+      short_long_kernel<<<(ns + 63) / 64, 64, 0, stream>>>(
           d_L, static_cast<int64_t>(n), d_row_ptr.ptr, d_branch.ptr, d_sign.ptr, d_short.ptr, ns,
           j, d_lmesh, loops);
       FH_HIP_TRY(hipGetLastError());
     }
-    product_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
+    product_kernel<<<static_cast<unsigned>((n + 63) / 64), 64, 0, stream>>>(
         d_L, static_cast<int64_t>(n), d_row_ptr.ptr, d_branch.ptr, d_sign.ptr, j, d_pv.ptr);
     FH_HIP_TRY(hipGetLastError());
     const int64_t rest = loops - j;
