@@ -357,6 +357,63 @@ def plain_copy(mib: int, repeats: int = 5) -> dict:
             "ms_per_copy": dt * 1e3, "gb_per_s": 8 * n / dt / 1e9}
 
 
+def library_gemm_ceiling(n: int = 8192, reps: int = 10) -> dict:
+    """The vendor library's bf16 GEMM (rocBLAS gemm_ex, bf16 inputs, FP32
+    accumulation and output, n x n x n, random data) on the same GPU: what
+    the bf16 matrix cores deliver in practice on this board, beside the
+    published dense peak the roofline is priced against."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    blas = C.CDLL("librocblas.so")
+    rng = np.random.default_rng(1)
+    bufs = []
+
+    def dev(nbytes, host=None):
+        ptr = C.c_void_p()
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        bufs.append(ptr)
+        if host is not None:
+            hip.hipMemcpy(ptr, host.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1)
+        return ptr
+
+    def bf16(shape):
+        x = rng.standard_normal(shape, dtype=np.float32)
+        return np.ascontiguousarray((x.view(np.uint32) >> 16).astype(np.uint16))
+
+    a = dev(n * n * 2, bf16((n, n)))
+    b = dev(n * n * 2, bf16((n, n)))
+    c = dev(n * n * 4, np.zeros((n, n), dtype=np.float32))
+    handle = C.c_void_p()
+    blas.rocblas_create_handle(C.byref(handle))
+    alpha, beta = C.c_float(1.0), C.c_float(0.0)
+    bf16_t, f32_t, none = 168, 151, 111
+
+    def gemm():
+        st = blas.rocblas_gemm_ex(handle, none, none, n, n, n, C.byref(alpha), a, bf16_t, n, b, bf16_t,
+                                  n, C.byref(beta), c, f32_t, n, c, f32_t, n, f32_t, 0, 0, 0)
+        if st != 0:
+            raise RuntimeError(f"rocblas_gemm_ex status {st}")
+
+    try:
+        for _ in range(3):
+            gemm()
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gemm()
+        hip.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        blas.rocblas_destroy_handle(handle)
+        for ptr in bufs:
+            hip.hipFree(ptr)
+    tflops = 2.0 * n ** 3 / dt / 1e12
+    return {"library": "rocBLAS gemm_ex bf16 -> fp32", "n": n, "tflops": tflops,
+            "frac_of_peak": tflops / PEAKS_TFLOPS["bf16"]}
+
+
 def model_pack(problem):
     from fasterhenry_amd import model
 
@@ -469,6 +526,34 @@ def main() -> int:
     achieved = upd_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
     unit_rate = upd_unit_flops / (upd_ms * 1e-3) / 1e12 if upd_ms > 0 else 0.0
 
+    # The same update kernel alone: one more profiled sweep with the
+    # frequencies in one group (no other stream's kernels beside its
+    # launches), and the vendor library's bf16 GEMM on the same GPU.
+    isolated = None
+    if args.precision != "fp64":
+        sched = system.schedule()
+        system.set_graphs(False)
+        system.set_schedule(sched["outer_block"], 1)
+        system.sweep_raw(freqs)
+        system.set_profiling(True)
+        _, iso = system.sweep_raw(freqs)
+        system.set_profiling(False)
+        system.set_schedule(sched["outer_block"], sched["sweep_parts"])
+        system.set_graphs(True)
+        iso = iso.as_dict()
+        iso_rate = iso["update_kernel_flops"] / (iso["update_kernel_ms"] * 1e-3) / 1e12
+        iso_unit = iso["update_kernel_unit_flops"] / (iso["update_kernel_ms"] * 1e-3) / 1e12
+        isolated = {"sweep": "one group of all frequencies on one stream", "achieved": iso_rate,
+                    "frac": iso_rate / peak, "matrix_core_tflops": iso_unit,
+                    "matrix_core_frac": iso_unit / peak,
+                    "avg_launch_ms": iso["update_kernel_ms"] / max(iso["update_kernel_launches"], 1)}
+        try:
+            ceiling = library_gemm_ceiling()
+            isolated["library_gemm"] = ceiling
+            isolated["matrix_core_vs_library"] = iso_unit / ceiling["tflops"]
+        except (OSError, RuntimeError) as e:
+            isolated["library_gemm"] = {"error": str(e)}
+
     extra = {}
     m = counts.internal_meshes
     if args.precision != "fp64" and not args.no_compare:
@@ -559,7 +644,8 @@ def main() -> int:
                      # duration stretches; this is every update flop of the
                      # timed steps over their whole wall time
                      "device_achieved": upd_flops / elapsed / 1e12,
-                     "device_frac": upd_flops / elapsed / 1e12 / peak},
+                     "device_frac": upd_flops / elapsed / 1e12 / peak,
+                     "isolated": isolated},
         "cpu_baseline": cpu,
         **extra,
     }

@@ -1012,7 +1012,8 @@ __global__ void dkey_kernel(const uint2 *__restrict__ queue, unsigned count, int
   idx[e] = e;
 }
 
-// One wave per (sorted queue entry e, direction q): dl[q count + e].
+// One wave per (sorted queue entry e, direction q): dl[q count + e], for the
+// heavy point quadratures and sampled pairs.
 __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__restrict__ F, int n,
                                                        const fh_filament *__restrict__ T,
                                                        const uint8_t *__restrict__ moving,
@@ -1029,6 +1030,8 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
     const uint2 qe = queue[sidx[e]];
     const int f = static_cast<int>(qe.x & 0x3fffffffu), g = static_cast<int>(qe.y);
     const uint32_t kind = qe.x >> 30;
+    // aligned entries go through their nine numbers (acontract_kernel)
+    if (kind == kAlignedKind) continue;
     const size_t qn = static_cast<size_t>(q) * n;
     double d = 0.0;
     if (moving[qn + f] || moving[qn + g]) {
@@ -1042,6 +1045,201 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
     if (lane == 0) dl[w] = d;
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Aligned pairs through their nine numbers.  An aligned pair's integral is a
+// function I(a) of its AlignedBars numbers a alone (lengths, widths,
+// heights, offsets; inductance/mod.rs:280-300), and with every branch
+// decided on the values each dual operation is linear in the tangents, so
+// the derivative along a direction q is sum_i dI/da_i da_i/dq.  I is even in
+// each offset (a reflection of both bars through a plane of bar a) and
+// symmetric in the two bars (with the offset negated), so the partials are
+// formed once per canonical a -- offsets by magnitude, the two bars in
+// (length, width, height) order -- and every (pair, direction) maps them
+// back and takes only its own tangent da/dq from the pair plan: nine
+// evaluations per distinct canonical a instead of one per pair and
+// direction (graded cross-sections repeat the same relative positions; on
+// the 6 156-filament spiral 1.63 M aligned entries have 0.49 M canonical
+// numbers).
+
+constexpr unsigned long long kNoAKey = ~0ull;
+
+__device__ __forceinline__ void aligned_values(const Aligned &ab, double (&w)[9]) {
+  w[0] = ab.length[0].v;
+  w[1] = ab.length[1].v;
+  w[2] = ab.width[0].v;
+  w[3] = ab.width[1].v;
+  w[4] = ab.height[0].v;
+  w[5] = ab.height[1].v;
+  w[6] = ab.offset[0].v;
+  w[7] = ab.offset[1].v;
+  w[8] = ab.offset[2].v;
+}
+
+// Canonical numbers of w and the map back: bit 0 set when the bars are
+// swapped, bits 1-3 when offset 0-2 is negative.
+__device__ __forceinline__ int canonical_aligned(const double (&w)[9], double (&c)[9]) {
+  const bool swap = w[1] < w[0] || (w[1] == w[0] && (w[3] < w[2] || (w[3] == w[2] && w[5] < w[4])));
+  c[0] = swap ? w[1] : w[0];
+  c[1] = swap ? w[0] : w[1];
+  c[2] = swap ? w[3] : w[2];
+  c[3] = swap ? w[2] : w[3];
+  c[4] = swap ? w[5] : w[4];
+  c[5] = swap ? w[4] : w[5];
+  int flags = swap ? 1 : 0;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    c[6 + t] = __builtin_fabs(w[6 + t]);
+    if (__builtin_signbit(w[6 + t])) flags |= 2 << t;
+  }
+  return flags;
+}
+
+// Sorted queue position e: the canonical aligned numbers of an aligned entry,
+// their hash (kNoAKey for the other kinds) and the map back.
+__global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__restrict__ queue,
+                            const unsigned *__restrict__ sidx, unsigned count, double *__restrict__ keys,
+                            unsigned long long *__restrict__ hash, unsigned *__restrict__ idx,
+                            int8_t *__restrict__ aflags) {
+  const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  idx[e] = e;
+  const uint2 qe = queue[sidx[e]];
+  unsigned long long h = kNoAKey;
+  int8_t fl = 0;
+  if ((qe.x >> 30) == kAlignedKind) {
+    const fh_filament zero = {};
+    Plan p;
+    plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, p);
+    double w[9], c[9];
+    aligned_values(p.ab, w);
+    fl = static_cast<int8_t>(canonical_aligned(w, c));
+    h = 0x9E3779B97F4A7C15ull;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      keys[static_cast<size_t>(e) * 9 + t] = c[t];
+      h ^= static_cast<unsigned long long>(__double_as_longlong(c[t]));
+      h ^= h >> 30;
+      h *= 0xbf58476d1ce4e5b9ull;
+      h ^= h >> 27;
+      h *= 0x94d049bb133111ebull;
+      h ^= h >> 31;
+    }
+    if (h == kNoAKey) h = kNoAKey - 1;
+  }
+  hash[e] = h;
+  aflags[e] = fl;
+}
+
+// head[j] = 1 where hash-sorted entry j starts a run of bit-equal keys.
+__global__ void ahead_kernel(const unsigned long long *__restrict__ sh, const unsigned *__restrict__ si,
+                             const double *__restrict__ keys, unsigned count, unsigned *__restrict__ head) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  unsigned hd = 0;
+  if (sh[j] != kNoAKey) {
+    hd = 1;
+    if (j > 0 && sh[j - 1] == sh[j]) {
+      bool same = true;
+      for (int t = 0; t < 9; ++t)
+        same = same && __double_as_longlong(keys[static_cast<size_t>(si[j]) * 9 + t]) ==
+                           __double_as_longlong(keys[static_cast<size_t>(si[j - 1]) * 9 + t]);
+      if (same) hd = 0;
+    }
+  }
+  head[j] = hd;
+}
+
+// urun[e] = the run (distinct key) of sorted queue position e, -1 for
+// entries of the other kinds; uq[u] = the position whose key run u
+// evaluates; *ucount = the number of runs.
+__global__ void arun_kernel(const unsigned long long *__restrict__ sh, const unsigned *__restrict__ si,
+                            const unsigned *__restrict__ head, const unsigned *__restrict__ run, unsigned count,
+                            int *__restrict__ urun, unsigned *__restrict__ uq, unsigned *__restrict__ ucount) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const unsigned e = si[j];
+  urun[e] = sh[j] == kNoAKey ? -1 : static_cast<int>(run[j]) - 1;
+  if (head[j]) uq[run[j] - 1] = e;
+  if (j == count - 1) *ucount = run[j];
+}
+
+// One wave per (run u, canonical number i): the integral's partial
+// derivative in c_i (ugrad[9 u + i]) and its value (uval[u]).
+__global__ void __launch_bounds__(64) agrad_kernel(const double *__restrict__ keys,
+                                                   const unsigned *__restrict__ uq,
+                                                   const unsigned *__restrict__ ucount,
+                                                   double *__restrict__ uval, double *__restrict__ ugrad) {
+  __shared__ Dual lds[kCloudCapacity];
+  __shared__ Frame stack[kStackCapacity];
+  const size_t total = static_cast<size_t>(*ucount) * 9;
+  for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const unsigned u = static_cast<unsigned>(w / 9);
+    const int i = static_cast<int>(w % 9);
+    const double *c = keys + static_cast<size_t>(uq[u]) * 9;
+    Aligned ab;
+    ab.length[0] = Dual{c[0], i == 0 ? 1.0 : 0.0};
+    ab.length[1] = Dual{c[1], i == 1 ? 1.0 : 0.0};
+    ab.width[0] = Dual{c[2], i == 2 ? 1.0 : 0.0};
+    ab.width[1] = Dual{c[3], i == 3 ? 1.0 : 0.0};
+    ab.height[0] = Dual{c[4], i == 4 ? 1.0 : 0.0};
+    ab.height[1] = Dual{c[5], i == 5 ? 1.0 : 0.0};
+    ab.offset[0] = Dual{c[6], i == 6 ? 1.0 : 0.0};
+    ab.offset[1] = Dual{c[7], i == 7 ? 1.0 : 0.0};
+    ab.offset[2] = Dual{c[8], i == 8 ? 1.0 : 0.0};
+    const Dual in = aligned_dispatch_wave(ab, stack, lds);
+    if (threadIdx.x == 0) {
+      ugrad[w] = in.d;
+      if (i == 0) uval[u] = in.v;
+    }
+    __syncthreads();
+  }
+}
+
+// One thread per (sorted aligned entry e, direction q): dl[q count + e] =
+// 1e-7 d(I sign(cos) / (area_a area_b)) with dI = sum_i dI/da_i da_i/dq (the
+// canonical partials mapped back through aflags[e]), as
+// the fill scales an aligned integral (deferred_eval).
+__global__ void __launch_bounds__(256) acontract_kernel(const fh_filament *__restrict__ F, int n,
+                                                        const fh_filament *__restrict__ T,
+                                                        const uint8_t *__restrict__ moving,
+                                                        const uint2 *__restrict__ queue,
+                                                        const unsigned *__restrict__ sidx, unsigned count, int P,
+                                                        const int *__restrict__ urun,
+                                                        const int8_t *__restrict__ aflags,
+                                                        const double *__restrict__ uval,
+                                                        const double *__restrict__ ugrad,
+                                                        double *__restrict__ dl, int *__restrict__ bad) {
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(count) * P) return;
+  const unsigned e = static_cast<unsigned>(t % count);
+  const int q = static_cast<int>(t / count);
+  const int u = urun[e];
+  if (u < 0) return;
+  const uint2 qe = queue[sidx[e]];
+  const int f = static_cast<int>(qe.x & 0x3fffffffu), g = static_cast<int>(qe.y);
+  const size_t qn = static_cast<size_t>(q) * n;
+  double d = 0.0;
+  if (moving[qn + f] || moving[qn + g]) {
+    Plan p;
+    plan_pair(F[f], T[qn + f], F[g], T[qn + g], p);
+    if (p.unsupported) atomicOr(bad, 2);
+    const double *gr = ugrad + static_cast<size_t>(u) * 9;
+    const Aligned &ab = p.ab;
+    // the canonical partials back on this pair's numbers
+    const int fl = aflags[e];
+    const bool sw = fl & 1;
+    const double s0 = fl & 2 ? -1.0 : 1.0, s1 = fl & 4 ? -1.0 : 1.0, s2 = fl & 8 ? -1.0 : 1.0;
+    const double di = (sw ? gr[1] : gr[0]) * ab.length[0].d + (sw ? gr[0] : gr[1]) * ab.length[1].d +
+                      (sw ? gr[3] : gr[2]) * ab.width[0].d + (sw ? gr[2] : gr[3]) * ab.width[1].d +
+                      (sw ? gr[5] : gr[4]) * ab.height[0].d + (sw ? gr[4] : gr[5]) * ab.height[1].d +
+                      s0 * gr[6] * ab.offset[0].d + s1 * gr[7] * ab.offset[1].d + s2 * gr[8] * ab.offset[2].d;
+    const Dual in{uval[u], di};
+    const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
+    d = (1.0e-7 * (in * scale)).d;
+  }
+  dl[t] = d;
 }
 
 // Sorted entries in blocks of 256 per direction (blockIdx.y): weighted
@@ -1276,6 +1474,49 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
     ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, dl.ptr,
                                                  flags.ptr);
     FH_HIP_TRY(hipGetLastError());
+    // aligned entries: nine partials per distinct set of aligned numbers,
+    // then one dot product per (entry, direction)
+    {
+      DeviceArray<double> akeys, uval, ugrad;
+      DeviceArray<unsigned long long> ahash, sahash;
+      DeviceArray<unsigned> aidx, saidx, head, run, uq, ucount;
+      DeviceArray<int> urun;
+      DeviceArray<int8_t> aflags;
+      DeviceArray<char> atemp;
+      if ((st = akeys.allocate(static_cast<size_t>(count) * 9)) || (st = ahash.allocate(count)) ||
+          (st = sahash.allocate(count)) || (st = aidx.allocate(count)) || (st = saidx.allocate(count)) ||
+          (st = head.allocate(count)) || (st = run.allocate(count)) || (st = uq.allocate(count)) ||
+          (st = ucount.allocate(1)) || (st = urun.allocate(count)) || (st = uval.allocate(count)) ||
+          (st = ugrad.allocate(static_cast<size_t>(count) * 9)) || (st = aflags.allocate(count)))
+        return st;
+      const unsigned eb = (count + 255) / 256;
+      akey_kernel<<<eb, 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, count, akeys.ptr, ahash.ptr, aidx.ptr,
+                                          aflags.ptr);
+      FH_HIP_TRY(hipGetLastError());
+      size_t sort_b = 0, scan_b = 0;
+      FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
+                                                   static_cast<int>(count), 0, 64, stream));
+      FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_b, head.ptr, run.ptr, static_cast<int>(count),
+                                                  stream));
+      size_t ab = std::max<size_t>(std::max(sort_b, scan_b), 1);
+      if ((st = atemp.allocate(ab))) return st;
+      FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(atemp.ptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
+                                                   static_cast<int>(count), 0, 64, stream));
+      ahead_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, akeys.ptr, count, head.ptr);
+      FH_HIP_TRY(hipGetLastError());
+      FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(atemp.ptr, ab, head.ptr, run.ptr, static_cast<int>(count),
+                                                  stream));
+      arun_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, head.ptr, run.ptr, count, urun.ptr, uq.ptr,
+                                          ucount.ptr);
+      FH_HIP_TRY(hipGetLastError());
+      agrad_kernel<<<wave_blocks(), 64, 0, stream>>>(akeys.ptr, uq.ptr, ucount.ptr, uval.ptr, ugrad.ptr);
+      FH_HIP_TRY(hipGetLastError());
+      acontract_kernel<<<static_cast<unsigned>((waves + 255) / 256), 256, 0, stream>>>(
+          d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr, aflags.ptr, uval.ptr, ugrad.ptr,
+          dl.ptr, flags.ptr);
+      FH_HIP_TRY(hipGetLastError());
+      FH_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
+    }
     const unsigned cblocks = (count + 255) / 256;
     if ((st = cpart.allocate(static_cast<size_t>(P) * cblocks * nout)) ||
         (st = res_d.allocate(static_cast<size_t>(P) * nout)))
