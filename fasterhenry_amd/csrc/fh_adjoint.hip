@@ -1013,12 +1013,14 @@ __global__ void dkey_kernel(const uint2 *__restrict__ queue, unsigned count, int
 }
 
 // One wave per (sorted queue entry e, direction q): dl[q count + e], for the
-// heavy point quadratures and sampled pairs.
+// entries not keyed by aligned numbers (sampled pairs, heavy point
+// quadratures of pairs that are not aligned).
 __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__restrict__ F, int n,
                                                        const fh_filament *__restrict__ T,
                                                        const uint8_t *__restrict__ moving,
                                                        const uint2 *__restrict__ queue,
                                                        const unsigned *__restrict__ sidx, unsigned count, int P,
+                                                       const int *__restrict__ urun,
                                                        double *__restrict__ dl, int *__restrict__ bad) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
@@ -1027,11 +1029,11 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
   for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
     const unsigned e = static_cast<unsigned>(w % count);
     const int q = static_cast<int>(w / count);
+    // keyed entries go through their canonical numbers (acontract_kernel)
+    if (urun[e] >= 0) continue;
     const uint2 qe = queue[sidx[e]];
     const int f = static_cast<int>(qe.x & 0x3fffffffu), g = static_cast<int>(qe.y);
     const uint32_t kind = qe.x >> 30;
-    // aligned entries go through their nine numbers (acontract_kernel)
-    if (kind == kAlignedKind) continue;
     const size_t qn = static_cast<size_t>(q) * n;
     double d = 0.0;
     if (moving[qn + f] || moving[qn + g]) {
@@ -1096,26 +1098,32 @@ __device__ __forceinline__ int canonical_aligned(const double (&w)[9], double (&
   return flags;
 }
 
-// Sorted queue position e: the canonical aligned numbers of an aligned entry,
-// their hash (kNoAKey for the other kinds) and the map back.
+// Sorted queue position e: for an aligned entry, or a heavy point
+// quadrature of an aligned pair, the canonical aligned numbers, their hash
+// with the kind (kNoAKey for every other entry) and the map back.
 __global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__restrict__ queue,
                             const unsigned *__restrict__ sidx, unsigned count, double *__restrict__ keys,
                             unsigned long long *__restrict__ hash, unsigned *__restrict__ idx,
-                            int8_t *__restrict__ aflags) {
+                            int8_t *__restrict__ aflags, int8_t *__restrict__ akind) {
   const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
   idx[e] = e;
   const uint2 qe = queue[sidx[e]];
+  const uint32_t kind = qe.x >> 30;
   unsigned long long h = kNoAKey;
   int8_t fl = 0;
-  if ((qe.x >> 30) == kAlignedKind) {
+  Plan p;
+  bool keyed = false;
+  if (kind == kAlignedKind || kind == kHeavyPoint) {
     const fh_filament zero = {};
-    Plan p;
     plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, p);
+    keyed = p.aligned;
+  }
+  if (keyed) {
     double w[9], c[9];
     aligned_values(p.ab, w);
     fl = static_cast<int8_t>(canonical_aligned(w, c));
-    h = 0x9E3779B97F4A7C15ull;
+    h = 0x9E3779B97F4A7C15ull ^ kind;
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       keys[static_cast<size_t>(e) * 9 + t] = c[t];
@@ -1130,18 +1138,20 @@ __global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__re
   }
   hash[e] = h;
   aflags[e] = fl;
+  akind[e] = static_cast<int8_t>(kind);
 }
 
 // head[j] = 1 where hash-sorted entry j starts a run of bit-equal keys.
 __global__ void ahead_kernel(const unsigned long long *__restrict__ sh, const unsigned *__restrict__ si,
-                             const double *__restrict__ keys, unsigned count, unsigned *__restrict__ head) {
+                             const double *__restrict__ keys, const int8_t *__restrict__ akind, unsigned count,
+                             unsigned *__restrict__ head) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   unsigned hd = 0;
   if (sh[j] != kNoAKey) {
     hd = 1;
     if (j > 0 && sh[j - 1] == sh[j]) {
-      bool same = true;
+      bool same = akind[si[j]] == akind[si[j - 1]];
       for (int t = 0; t < 9; ++t)
         same = same && __double_as_longlong(keys[static_cast<size_t>(si[j]) * 9 + t]) ==
                            __double_as_longlong(keys[static_cast<size_t>(si[j - 1]) * 9 + t]);
@@ -1165,41 +1175,98 @@ __global__ void arun_kernel(const unsigned long long *__restrict__ sh, const uns
   if (j == count - 1) *ucount = run[j];
 }
 
-// One wave per (run u, canonical number i): the integral's partial
-// derivative in c_i (ugrad[9 u + i]) and its value (uval[u]).
-__global__ void __launch_bounds__(64) agrad_kernel(const double *__restrict__ keys,
+// Bar of the canonical frame: centre (x, y, z), axes along x, y, z, full
+// extents (l, w, h); tangent 1 on the one number `i` of nine: the centre's
+// three at ix, ix + 1, ix + 2 (ix < 0: a fixed centre), the extents at il,
+// iw, ih.
+__device__ __forceinline__ Bar canonical_bar(double x, double y, double z, double l, double w, double h,
+                                             int ix, int il, int iw, int ih, int i) {
+  Bar b;
+  const bool moves = ix >= 0;
+  b.centre = V3{Dual{x, moves && i == ix ? 1.0 : 0.0}, Dual{y, moves && i == ix + 1 ? 1.0 : 0.0},
+                Dual{z, moves && i == ix + 2 ? 1.0 : 0.0}};
+  b.axes[0] = V3{cst(1.0), cst(0.0), cst(0.0)};
+  b.axes[1] = V3{cst(0.0), cst(1.0), cst(0.0)};
+  b.axes[2] = V3{cst(0.0), cst(0.0), cst(1.0)};
+  b.half[0] = Dual{0.5 * l, i == il ? 0.5 : 0.0};
+  b.half[1] = Dual{0.5 * w, i == iw ? 0.5 : 0.0};
+  b.half[2] = Dual{0.5 * h, i == ih ? 0.5 : 0.0};
+  return b;
+}
+
+// One wave per (run u, canonical number i): the partial derivative in c_i
+// (ugrad[9 u + i]) and the value (uval[u]) of the run's integral -- for an
+// aligned entry I(c) of the aligned dispatch (scaled by the pair later); for
+// a heavy point quadrature 1e-7 x the point sum of two bars placed in the
+// canonical frame (bar a at the origin, bar b at the offsets, both along x,
+// y, z), with the quadrature orders of the run's own pair mapped onto them.
+__global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict__ F,
+                                                   const uint2 *__restrict__ queue,
+                                                   const unsigned *__restrict__ sidx,
+                                                   const double *__restrict__ keys,
+                                                   const int8_t *__restrict__ aflags,
                                                    const unsigned *__restrict__ uq,
                                                    const unsigned *__restrict__ ucount,
                                                    double *__restrict__ uval, double *__restrict__ ugrad) {
   __shared__ Dual lds[kCloudCapacity];
-  __shared__ Frame stack[kStackCapacity];
+  __shared__ DeferredScratch scratch_lds;
+  const int lane = static_cast<int>(threadIdx.x);
   const size_t total = static_cast<size_t>(*ucount) * 9;
   for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
     const unsigned u = static_cast<unsigned>(w / 9);
     const int i = static_cast<int>(w % 9);
-    const double *c = keys + static_cast<size_t>(uq[u]) * 9;
-    Aligned ab;
-    ab.length[0] = Dual{c[0], i == 0 ? 1.0 : 0.0};
-    ab.length[1] = Dual{c[1], i == 1 ? 1.0 : 0.0};
-    ab.width[0] = Dual{c[2], i == 2 ? 1.0 : 0.0};
-    ab.width[1] = Dual{c[3], i == 3 ? 1.0 : 0.0};
-    ab.height[0] = Dual{c[4], i == 4 ? 1.0 : 0.0};
-    ab.height[1] = Dual{c[5], i == 5 ? 1.0 : 0.0};
-    ab.offset[0] = Dual{c[6], i == 6 ? 1.0 : 0.0};
-    ab.offset[1] = Dual{c[7], i == 7 ? 1.0 : 0.0};
-    ab.offset[2] = Dual{c[8], i == 8 ? 1.0 : 0.0};
-    const Dual in = aligned_dispatch_wave(ab, stack, lds);
-    if (threadIdx.x == 0) {
-      ugrad[w] = in.d;
-      if (i == 0) uval[u] = in.v;
+    const unsigned pos = uq[u];
+    const double *c = keys + static_cast<size_t>(pos) * 9;
+    const uint2 qe = queue[sidx[pos]];
+    Dual v;
+    if ((qe.x >> 30) == kAlignedKind) {
+      Aligned ab;
+      ab.length[0] = Dual{c[0], i == 0 ? 1.0 : 0.0};
+      ab.length[1] = Dual{c[1], i == 1 ? 1.0 : 0.0};
+      ab.width[0] = Dual{c[2], i == 2 ? 1.0 : 0.0};
+      ab.width[1] = Dual{c[3], i == 3 ? 1.0 : 0.0};
+      ab.height[0] = Dual{c[4], i == 4 ? 1.0 : 0.0};
+      ab.height[1] = Dual{c[5], i == 5 ? 1.0 : 0.0};
+      ab.offset[0] = Dual{c[6], i == 6 ? 1.0 : 0.0};
+      ab.offset[1] = Dual{c[7], i == 7 ? 1.0 : 0.0};
+      ab.offset[2] = Dual{c[8], i == 8 ? 1.0 : 0.0};
+      v = aligned_dispatch_wave(ab, scratch_lds.stack, lds);
+    } else {
+      // the run's pair for its orders: b's own width/height orders go to
+      // the canonical y/z axes it is aligned with, and the bars trade
+      // places when the canonical numbers swap them
+      const fh_filament zero = {};
+      Plan r;
+      plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, r);
+      const bool sameb = __builtin_fabs(dot(r.a.axes[1], r.b.axes[1]).v) > 1.0 - 1e-12;
+      const int obc0 = r.ob[0], obc1 = sameb ? r.ob[1] : r.ob[2], obc2 = sameb ? r.ob[2] : r.ob[1];
+      const bool sw = aflags[pos] & 1;
+      Plan p;
+      p.oa[0] = sw ? obc0 : r.oa[0];
+      p.oa[1] = sw ? obc1 : r.oa[1];
+      p.oa[2] = sw ? obc2 : r.oa[2];
+      p.ob[0] = sw ? r.oa[0] : obc0;
+      p.ob[1] = sw ? r.oa[1] : obc1;
+      p.ob[2] = sw ? r.oa[2] : obc2;
+      p.a = canonical_bar(0.0, 0.0, 0.0, c[0], c[2], c[4], -1, 0, 2, 4, i);
+      p.b = canonical_bar(c[6], c[7], c[8], c[1], c[3], c[5], 6, 1, 3, 5, i);
+      p.cos = cst(1.0);
+      p.point = true;
+      bool ok;
+      v = deferred_eval(p, kHeavyPoint, lds, scratch_lds, lane, ok);
+    }
+    if (lane == 0) {
+      ugrad[w] = v.d;
+      if (i == 0) uval[u] = v.v;
     }
     __syncthreads();
   }
 }
 
-// One thread per (sorted aligned entry e, direction q): dl[q count + e] =
-// 1e-7 d(I sign(cos) / (area_a area_b)) with dI = sum_i dI/da_i da_i/dq (the
-// canonical partials mapped back through aflags[e]), as
+// One thread per (sorted keyed entry e, direction q): dl[q count + e] =
+// 1e-7 d(I sign(cos) / (area_a area_b)) for an aligned entry, d(I cos) for
+// a heavy point quadrature, with dI = sum_i dI/da_i da_i/dq (the canonical
+// partials mapped back through aflags[e]), as
 // the fill scales an aligned integral (deferred_eval).
 __global__ void __launch_bounds__(256) acontract_kernel(const fh_filament *__restrict__ F, int n,
                                                         const fh_filament *__restrict__ T,
@@ -1236,8 +1303,12 @@ __global__ void __launch_bounds__(256) acontract_kernel(const fh_filament *__res
                       (sw ? gr[5] : gr[4]) * ab.height[0].d + (sw ? gr[4] : gr[5]) * ab.height[1].d +
                       s0 * gr[6] * ab.offset[0].d + s1 * gr[7] * ab.offset[1].d + s2 * gr[8] * ab.offset[2].d;
     const Dual in{uval[u], di};
-    const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
-    d = (1.0e-7 * (in * scale)).d;
+    if ((qe.x >> 30) == kAlignedKind) {
+      const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
+      d = (1.0e-7 * (in * scale)).d;
+    } else {
+      d = (in * p.cos).d;  // heavy point sum: 1e-7 already in
+    }
   }
   dl[t] = d;
 }
@@ -1470,53 +1541,54 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
     FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(temp.ptr, tb, key.ptr, skey.ptr, idx.ptr, sidx.ptr,
                                                  static_cast<int>(count), 0, end_bit, stream));
     const size_t waves = static_cast<size_t>(count) * P;
-    const unsigned dblocks = static_cast<unsigned>(std::min<size_t>(waves, wave_blocks()));
-    ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, dl.ptr,
-                                                 flags.ptr);
+    // entries keyed by their canonical aligned numbers (aligned entries and
+    // heavy point quadratures of aligned pairs): nine partials per distinct
+    // key, then one dot product per (entry, direction); the other entries
+    // one wave per (entry, direction)
+    DeviceArray<double> akeys, uval, ugrad;
+    DeviceArray<unsigned long long> ahash, sahash;
+    DeviceArray<unsigned> aidx, saidx, head, run, uq, ucount;
+    DeviceArray<int> urun;
+    DeviceArray<int8_t> aflags, akind;
+    DeviceArray<char> atemp;
+    if ((st = akeys.allocate(static_cast<size_t>(count) * 9)) || (st = ahash.allocate(count)) ||
+        (st = sahash.allocate(count)) || (st = aidx.allocate(count)) || (st = saidx.allocate(count)) ||
+        (st = head.allocate(count)) || (st = run.allocate(count)) || (st = uq.allocate(count)) ||
+        (st = ucount.allocate(1)) || (st = urun.allocate(count)) || (st = uval.allocate(count)) ||
+        (st = ugrad.allocate(static_cast<size_t>(count) * 9)) || (st = aflags.allocate(count)) ||
+        (st = akind.allocate(count)))
+      return st;
+    const unsigned eb = (count + 255) / 256;
+    akey_kernel<<<eb, 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, count, akeys.ptr, ahash.ptr, aidx.ptr,
+                                        aflags.ptr, akind.ptr);
     FH_HIP_TRY(hipGetLastError());
-    // aligned entries: nine partials per distinct set of aligned numbers,
-    // then one dot product per (entry, direction)
-    {
-      DeviceArray<double> akeys, uval, ugrad;
-      DeviceArray<unsigned long long> ahash, sahash;
-      DeviceArray<unsigned> aidx, saidx, head, run, uq, ucount;
-      DeviceArray<int> urun;
-      DeviceArray<int8_t> aflags;
-      DeviceArray<char> atemp;
-      if ((st = akeys.allocate(static_cast<size_t>(count) * 9)) || (st = ahash.allocate(count)) ||
-          (st = sahash.allocate(count)) || (st = aidx.allocate(count)) || (st = saidx.allocate(count)) ||
-          (st = head.allocate(count)) || (st = run.allocate(count)) || (st = uq.allocate(count)) ||
-          (st = ucount.allocate(1)) || (st = urun.allocate(count)) || (st = uval.allocate(count)) ||
-          (st = ugrad.allocate(static_cast<size_t>(count) * 9)) || (st = aflags.allocate(count)))
-        return st;
-      const unsigned eb = (count + 255) / 256;
-      akey_kernel<<<eb, 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, count, akeys.ptr, ahash.ptr, aidx.ptr,
-                                          aflags.ptr);
-      FH_HIP_TRY(hipGetLastError());
-      size_t sort_b = 0, scan_b = 0;
-      FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
-                                                   static_cast<int>(count), 0, 64, stream));
-      FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_b, head.ptr, run.ptr, static_cast<int>(count),
-                                                  stream));
-      size_t ab = std::max<size_t>(std::max(sort_b, scan_b), 1);
-      if ((st = atemp.allocate(ab))) return st;
-      FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(atemp.ptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
-                                                   static_cast<int>(count), 0, 64, stream));
-      ahead_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, akeys.ptr, count, head.ptr);
-      FH_HIP_TRY(hipGetLastError());
-      FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(atemp.ptr, ab, head.ptr, run.ptr, static_cast<int>(count),
-                                                  stream));
-      arun_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, head.ptr, run.ptr, count, urun.ptr, uq.ptr,
-                                          ucount.ptr);
-      FH_HIP_TRY(hipGetLastError());
-      agrad_kernel<<<wave_blocks(), 64, 0, stream>>>(akeys.ptr, uq.ptr, ucount.ptr, uval.ptr, ugrad.ptr);
-      FH_HIP_TRY(hipGetLastError());
-      acontract_kernel<<<static_cast<unsigned>((waves + 255) / 256), 256, 0, stream>>>(
-          d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr, aflags.ptr, uval.ptr, ugrad.ptr,
-          dl.ptr, flags.ptr);
-      FH_HIP_TRY(hipGetLastError());
-      FH_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed on return
-    }
+    size_t sort_b = 0, scan_b = 0;
+    FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
+                                                 static_cast<int>(count), 0, 64, stream));
+    FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, scan_b, head.ptr, run.ptr, static_cast<int>(count),
+                                                stream));
+    size_t ab = std::max<size_t>(std::max(sort_b, scan_b), 1);
+    if ((st = atemp.allocate(ab))) return st;
+    FH_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(atemp.ptr, sort_b, ahash.ptr, sahash.ptr, aidx.ptr, saidx.ptr,
+                                                 static_cast<int>(count), 0, 64, stream));
+    ahead_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, akeys.ptr, akind.ptr, count, head.ptr);
+    FH_HIP_TRY(hipGetLastError());
+    FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(atemp.ptr, ab, head.ptr, run.ptr, static_cast<int>(count),
+                                                stream));
+    arun_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, head.ptr, run.ptr, count, urun.ptr, uq.ptr,
+                                        ucount.ptr);
+    FH_HIP_TRY(hipGetLastError());
+    const unsigned dblocks = static_cast<unsigned>(std::min<size_t>(waves, wave_blocks()));
+    ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr,
+                                                 dl.ptr, flags.ptr);
+    FH_HIP_TRY(hipGetLastError());
+    agrad_kernel<<<wave_blocks(), 64, 0, stream>>>(d_F, queue.ptr, sidx.ptr, akeys.ptr, aflags.ptr, uq.ptr,
+                                                   ucount.ptr, uval.ptr, ugrad.ptr);
+    FH_HIP_TRY(hipGetLastError());
+    acontract_kernel<<<static_cast<unsigned>((waves + 255) / 256), 256, 0, stream>>>(
+        d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr, aflags.ptr, uval.ptr, ugrad.ptr,
+        dl.ptr, flags.ptr);
+    FH_HIP_TRY(hipGetLastError());
     const unsigned cblocks = (count + 255) / 256;
     if ((st = cpart.allocate(static_cast<size_t>(P) * cblocks * nout)) ||
         (st = res_d.allocate(static_cast<size_t>(P) * nout)))
@@ -1527,6 +1599,7 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
     dreduce_kernel<<<P * nout, 256, 0, stream>>>(cpart.ptr, static_cast<int>(cblocks), nout, res_d.ptr);
     FH_HIP_TRY(hipGetLastError());
     FH_HIP_TRY(hipMemcpyAsync(hd.data(), res_d.ptr, hd.size() * sizeof(double2), hipMemcpyDeviceToHost, stream));
+    FH_HIP_TRY(hipStreamSynchronize(stream));  // the temporaries are freed at the end of this scope
   }
   int hflags = 0;
   FH_HIP_TRY(hipMemcpyAsync(&hflags, flags.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
