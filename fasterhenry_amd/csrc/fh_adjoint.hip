@@ -51,7 +51,7 @@ __constant__ double a_gw[kMaxRule + 1][kMaxRule];
 __constant__ double a_point_th[kPointMaxOrder];
 __constant__ double a_sample_th[kSampleMaxOrder];
 
-enum Kind : uint32_t { kHeavyPoint = 1u, kAlignedKind = 2u, kSampled = 3u };
+enum Kind : uint32_t { kLightPoint = 0u, kHeavyPoint = 1u, kAlignedKind = 2u, kSampled = 3u };
 
 // ---------------------------------------------------------------------------
 // Dual numbers.
@@ -981,10 +981,13 @@ __global__ void __launch_bounds__(256, 3) dtile_kernel(const fh_filament *__rest
       plan_pair(F[f], T[qn + f], F[g], T[qn + g], p);
       if (p.unsupported) atomicOr(flags, 2);
       if (!p.orthogonal) {
-        if (p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals) {
+        const bool light = p.point && p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]) <= kInlineEvals;
+        if (light && !p.aligned) {
           if (moves) d = (1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos)).d;
         } else if (q == 0) {
-          kind = p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled);
+          // aligned light point quadratures go through their canonical
+          // aligned numbers, with the queued pairs
+          kind = light ? kLightPoint : (p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled));
           queued = true;
         }
       }
@@ -1020,17 +1023,19 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
                                                        const uint8_t *__restrict__ moving,
                                                        const uint2 *__restrict__ queue,
                                                        const unsigned *__restrict__ sidx, unsigned count, int P,
-                                                       const int *__restrict__ urun,
+                                                       const unsigned *__restrict__ hsorted,
+                                                       const unsigned *__restrict__ unkeyed_begin,
                                                        double *__restrict__ dl, int *__restrict__ bad) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
-  const size_t total = static_cast<size_t>(count) * P;
+  // the entries without aligned numbers close the hash order
+  const unsigned beg = *unkeyed_begin;
+  const size_t nun = count - beg;
+  const size_t total = nun * P;
   for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
-    const unsigned e = static_cast<unsigned>(w % count);
-    const int q = static_cast<int>(w / count);
-    // keyed entries go through their canonical numbers (acontract_kernel)
-    if (urun[e] >= 0) continue;
+    const unsigned e = hsorted[beg + static_cast<unsigned>(w % nun)];
+    const int q = static_cast<int>(w / nun);
     const uint2 qe = queue[sidx[e]];
     const int f = static_cast<int>(qe.x & 0x3fffffffu), g = static_cast<int>(qe.y);
     const uint32_t kind = qe.x >> 30;
@@ -1044,7 +1049,7 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
       if (lane == 0 && !ok) atomicOr(bad, 1);
       if (lane == 0 && p.unsupported) atomicOr(bad, 2);
     }
-    if (lane == 0) dl[w] = d;
+    if (lane == 0) dl[static_cast<size_t>(q) * count + e] = d;
     __syncthreads();
   }
 }
@@ -1114,7 +1119,7 @@ __global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__re
   int8_t fl = 0;
   Plan p;
   bool keyed = false;
-  if (kind == kAlignedKind || kind == kHeavyPoint) {
+  if (kind != kSampled) {
     const fh_filament zero = {};
     plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, p);
     keyed = p.aligned;
@@ -1134,7 +1139,9 @@ __global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__re
       h *= 0x94d049bb133111ebull;
       h ^= h >> 31;
     }
-    if (h == kNoAKey) h = kNoAKey - 1;
+    // the kind in the top two bits: the runs of each kind are contiguous in
+    // hash order, light point quadratures first (kNoAKey sorts last)
+    h = (static_cast<unsigned long long>(kind) << 62) | (h >> 2);
   }
   hash[e] = h;
   aflags[e] = fl;
@@ -1166,13 +1173,18 @@ __global__ void ahead_kernel(const unsigned long long *__restrict__ sh, const un
 // evaluates; *ucount = the number of runs.
 __global__ void arun_kernel(const unsigned long long *__restrict__ sh, const unsigned *__restrict__ si,
                             const unsigned *__restrict__ head, const unsigned *__restrict__ run, unsigned count,
-                            int *__restrict__ urun, unsigned *__restrict__ uq, unsigned *__restrict__ ucount) {
+                            int *__restrict__ urun, unsigned *__restrict__ uq, unsigned *__restrict__ ucount,
+                            unsigned *__restrict__ nlight, unsigned *__restrict__ unkeyed_begin) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const unsigned e = si[j];
   urun[e] = sh[j] == kNoAKey ? -1 : static_cast<int>(run[j]) - 1;
   if (head[j]) uq[run[j] - 1] = e;
   if (j == count - 1) *ucount = run[j];
+  // the last light entry in hash order closes the light runs
+  const bool light = sh[j] >> 62 == kLightPoint;
+  if (light && (j == count - 1 || sh[j + 1] >> 62 != kLightPoint)) *nlight = run[j];
+  if (sh[j] == kNoAKey && (j == 0 || sh[j - 1] != kNoAKey)) *unkeyed_begin = j;
 }
 
 // Bar of the canonical frame: centre (x, y, z), axes along x, y, z, full
@@ -1194,12 +1206,36 @@ __device__ __forceinline__ Bar canonical_bar(double x, double y, double z, doubl
   return b;
 }
 
-// One wave per (run u, canonical number i): the partial derivative in c_i
-// (ugrad[9 u + i]) and the value (uval[u]) of the run's integral -- for an
-// aligned entry I(c) of the aligned dispatch (scaled by the pair later); for
-// a heavy point quadrature 1e-7 x the point sum of two bars placed in the
-// canonical frame (bar a at the origin, bar b at the offsets, both along x,
-// y, z), with the quadrature orders of the run's own pair mapped onto them.
+// The point quadrature of run u's pair in the canonical frame (bar a at the
+// origin, bar b at the offsets, both along x, y, z) with tangent 1 on
+// canonical number i: the orders are the run's own pair's, b's own width and
+// height orders moved to the canonical y/z axes it is aligned with, the bars
+// trading places when the canonical numbers swap them.
+__device__ __forceinline__ void canonical_point_plan(const fh_filament *__restrict__ F, uint2 qe, int flags,
+                                                     const double *c, int i, Plan &p) {
+  const fh_filament zero = {};
+  Plan r;
+  plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, r);
+  const bool sameb = __builtin_fabs(dot(r.a.axes[1], r.b.axes[1]).v) > 1.0 - 1e-12;
+  const int obc0 = r.ob[0], obc1 = sameb ? r.ob[1] : r.ob[2], obc2 = sameb ? r.ob[2] : r.ob[1];
+  const bool sw = flags & 1;
+  p.oa[0] = sw ? obc0 : r.oa[0];
+  p.oa[1] = sw ? obc1 : r.oa[1];
+  p.oa[2] = sw ? obc2 : r.oa[2];
+  p.ob[0] = sw ? r.oa[0] : obc0;
+  p.ob[1] = sw ? r.oa[1] : obc1;
+  p.ob[2] = sw ? r.oa[2] : obc2;
+  p.a = canonical_bar(0.0, 0.0, 0.0, c[0], c[2], c[4], -1, 0, 2, 4, i);
+  p.b = canonical_bar(c[6], c[7], c[8], c[1], c[3], c[5], 6, 1, 3, 5, i);
+  p.cos = cst(1.0);
+  p.point = true;
+}
+
+// One wave per (run u, canonical number i), u in [first, *ucount): the
+// partial derivative in c_i (ugrad[9 u + i]) and the value (uval[u]) of the
+// run's integral -- for an aligned entry I(c) of the aligned dispatch
+// (scaled by the pair later); for a heavy point quadrature 1e-7 x the point
+// sum of the canonical plan.
 __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict__ F,
                                                    const uint2 *__restrict__ queue,
                                                    const unsigned *__restrict__ sidx,
@@ -1207,12 +1243,13 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
                                                    const int8_t *__restrict__ aflags,
                                                    const unsigned *__restrict__ uq,
                                                    const unsigned *__restrict__ ucount,
+                                                   const unsigned *__restrict__ first,
                                                    double *__restrict__ uval, double *__restrict__ ugrad) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
-  const size_t total = static_cast<size_t>(*ucount) * 9;
-  for (size_t w = blockIdx.x; w < total; w += gridDim.x) {
+  const size_t begin = static_cast<size_t>(*first) * 9, total = static_cast<size_t>(*ucount) * 9;
+  for (size_t w = begin + blockIdx.x; w < total; w += gridDim.x) {
     const unsigned u = static_cast<unsigned>(w / 9);
     const int i = static_cast<int>(w % 9);
     const unsigned pos = uq[u];
@@ -1232,26 +1269,8 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
       ab.offset[2] = Dual{c[8], i == 8 ? 1.0 : 0.0};
       v = aligned_dispatch_wave(ab, scratch_lds.stack, lds);
     } else {
-      // the run's pair for its orders: b's own width/height orders go to
-      // the canonical y/z axes it is aligned with, and the bars trade
-      // places when the canonical numbers swap them
-      const fh_filament zero = {};
-      Plan r;
-      plan_pair(F[qe.x & 0x3fffffffu], zero, F[qe.y], zero, r);
-      const bool sameb = __builtin_fabs(dot(r.a.axes[1], r.b.axes[1]).v) > 1.0 - 1e-12;
-      const int obc0 = r.ob[0], obc1 = sameb ? r.ob[1] : r.ob[2], obc2 = sameb ? r.ob[2] : r.ob[1];
-      const bool sw = aflags[pos] & 1;
       Plan p;
-      p.oa[0] = sw ? obc0 : r.oa[0];
-      p.oa[1] = sw ? obc1 : r.oa[1];
-      p.oa[2] = sw ? obc2 : r.oa[2];
-      p.ob[0] = sw ? r.oa[0] : obc0;
-      p.ob[1] = sw ? r.oa[1] : obc1;
-      p.ob[2] = sw ? r.oa[2] : obc2;
-      p.a = canonical_bar(0.0, 0.0, 0.0, c[0], c[2], c[4], -1, 0, 2, 4, i);
-      p.b = canonical_bar(c[6], c[7], c[8], c[1], c[3], c[5], 6, 1, 3, 5, i);
-      p.cos = cst(1.0);
-      p.point = true;
+      canonical_point_plan(F, qe, aflags[pos], c, i, p);
       bool ok;
       v = deferred_eval(p, kHeavyPoint, lds, scratch_lds, lane, ok);
     }
@@ -1260,6 +1279,31 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
       if (i == 0) uval[u] = v.v;
     }
     __syncthreads();
+  }
+}
+
+// One thread per (run u, canonical number i), u < *nlight (the light point
+// quadratures): 1e-7 x the serial point sum of the canonical plan, as the
+// tile kernels evaluate a light pair.
+__global__ void __launch_bounds__(256) agrad_light_kernel(const fh_filament *__restrict__ F,
+                                                          const uint2 *__restrict__ queue,
+                                                          const unsigned *__restrict__ sidx,
+                                                          const double *__restrict__ keys,
+                                                          const int8_t *__restrict__ aflags,
+                                                          const unsigned *__restrict__ uq,
+                                                          const unsigned *__restrict__ nlight,
+                                                          double *__restrict__ uval, double *__restrict__ ugrad) {
+  const size_t total = static_cast<size_t>(*nlight) * 9;
+  for (size_t w = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; w < total;
+       w += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const unsigned u = static_cast<unsigned>(w / 9);
+    const int i = static_cast<int>(w % 9);
+    const unsigned pos = uq[u];
+    Plan p;
+    canonical_point_plan(F, queue[sidx[pos]], aflags[pos], keys + static_cast<size_t>(pos) * 9, i, p);
+    const Dual v = 1.0e-7 * point_sum_serial(p.a, p.oa, p.b, p.ob);
+    ugrad[w] = v.d;
+    if (i == 0) uval[u] = v.v;
   }
 }
 
@@ -1307,7 +1351,7 @@ __global__ void __launch_bounds__(256) acontract_kernel(const fh_filament *__res
       const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
       d = (1.0e-7 * (in * scale)).d;
     } else {
-      d = (in * p.cos).d;  // heavy point sum: 1e-7 already in
+      d = (in * p.cos).d;  // point sums: 1e-7 already in
     }
   }
   dl[t] = d;
@@ -1547,7 +1591,7 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
     // one wave per (entry, direction)
     DeviceArray<double> akeys, uval, ugrad;
     DeviceArray<unsigned long long> ahash, sahash;
-    DeviceArray<unsigned> aidx, saidx, head, run, uq, ucount;
+    DeviceArray<unsigned> aidx, saidx, head, run, uq, ucount, nlight, ubeg;
     DeviceArray<int> urun;
     DeviceArray<int8_t> aflags, akind;
     DeviceArray<char> atemp;
@@ -1556,8 +1600,11 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
         (st = head.allocate(count)) || (st = run.allocate(count)) || (st = uq.allocate(count)) ||
         (st = ucount.allocate(1)) || (st = urun.allocate(count)) || (st = uval.allocate(count)) ||
         (st = ugrad.allocate(static_cast<size_t>(count) * 9)) || (st = aflags.allocate(count)) ||
-        (st = akind.allocate(count)))
+        (st = akind.allocate(count)) || (st = nlight.allocate(1)) || (st = ubeg.allocate(1)))
       return st;
+    FH_HIP_TRY(hipMemsetAsync(nlight.ptr, 0, sizeof(unsigned), stream));
+    // every entry keyed: the unkeyed range is empty
+    FH_HIP_TRY(hipMemcpyAsync(ubeg.ptr, &count, sizeof(unsigned), hipMemcpyHostToDevice, stream));
     const unsigned eb = (count + 255) / 256;
     akey_kernel<<<eb, 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, count, akeys.ptr, ahash.ptr, aidx.ptr,
                                         aflags.ptr, akind.ptr);
@@ -1576,14 +1623,17 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
     FH_HIP_TRY(hipcub::DeviceScan::InclusiveSum(atemp.ptr, ab, head.ptr, run.ptr, static_cast<int>(count),
                                                 stream));
     arun_kernel<<<eb, 256, 0, stream>>>(sahash.ptr, saidx.ptr, head.ptr, run.ptr, count, urun.ptr, uq.ptr,
-                                        ucount.ptr);
+                                        ucount.ptr, nlight.ptr, ubeg.ptr);
     FH_HIP_TRY(hipGetLastError());
     const unsigned dblocks = static_cast<unsigned>(std::min<size_t>(waves, wave_blocks()));
-    ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr,
-                                                 dl.ptr, flags.ptr);
+    ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P,
+                                                 saidx.ptr, ubeg.ptr, dl.ptr, flags.ptr);
     FH_HIP_TRY(hipGetLastError());
     agrad_kernel<<<wave_blocks(), 64, 0, stream>>>(d_F, queue.ptr, sidx.ptr, akeys.ptr, aflags.ptr, uq.ptr,
-                                                   ucount.ptr, uval.ptr, ugrad.ptr);
+                                                   ucount.ptr, nlight.ptr, uval.ptr, ugrad.ptr);
+    FH_HIP_TRY(hipGetLastError());
+    agrad_light_kernel<<<wave_blocks(), 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, akeys.ptr, aflags.ptr,
+                                                          uq.ptr, nlight.ptr, uval.ptr, ugrad.ptr);
     FH_HIP_TRY(hipGetLastError());
     acontract_kernel<<<static_cast<unsigned>((waves + 255) / 256), 256, 0, stream>>>(
         d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P, urun.ptr, aflags.ptr, uval.ptr, ugrad.ptr,
