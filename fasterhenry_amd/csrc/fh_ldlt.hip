@@ -617,6 +617,11 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
         }
     cur = nxt;
   }
+  // The results go in place: waves 0 and 2 (1 and 3) read the same 32 rows
+  // over all 64 columns and each writes half of those columns.  Every
+  // wave's loads come before any wave's stores, however far the waves have
+  // drifted apart under other work on the compute unit.
+  __syncthreads();
 #pragma unroll
   for (int ib = 0; ib < 2; ++ib)
 #pragma unroll

@@ -1078,16 +1078,8 @@ fh_status fh_solve_batch(const fh_problem *problems, size_t count, fh_solver_cho
   std::vector<size_t> order(count);
   std::iota(order.begin(), order.end(), size_t{0});
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
-  // Plain-FP64 batches run on one worker.  With three or four workers
-  // factorizing FP64 systems side by side, about one batch in four of 48
-  // problems came back with one frequency of one problem wrong (a 32 x 32
-  // block of a panel or trailing-update result computed differently), never
-  // with one worker, never for concurrent single-system FP64 sweeps and never
-  // for the refined form (bit-identical to its own solves in every run); the
-  // cause is not found, so the FP64 form is kept to the schedule that was
-  // always correct.
   const size_t want = concurrency > 0 ? static_cast<size_t>(concurrency) : 4;
-  const size_t workers = std::min<size_t>(count, precision == FH_FP64_PLAIN ? 1 : want);
+  const size_t workers = std::min<size_t>(count, want);
   int device = 0;
   FH_HIP_TRY(hipGetDevice(&device));
   struct Outcome {

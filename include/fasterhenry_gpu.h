@@ -315,8 +315,8 @@ fh_status fh_solve(const fh_problem *problem, fh_solver_choice choice, fh_precis
 /* A batch of independent problems swept over the same K frequencies -- the
  * work of one solve() per problem (solve.rs:931-938), scheduled together:
  * systems are assembled and swept concurrently on the device by
- * `concurrency` workers (0: the library's choice; FH_FP64_PLAIN batches use
- * one worker), largest problems first, each on its own stream.  Each
+ * `concurrency` workers (0: the library's choice), largest problems first,
+ * each on its own stream.  Each
  * problem's result is identical to its own
  * fh_solve.  z_out: problem b's K x p_b x p_b {re, im} values start at
  * 2 K sum_{a<b} p_a^2 doubles.  infos (may be NULL): one fh_sweep_info per

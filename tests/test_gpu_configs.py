@@ -171,6 +171,21 @@ def test_full_batch_equals_one_solve_per_problem():
     print("batch: every problem bit-identical to its own solve", flush=True)
 
 
+@pytest.mark.timeout(300)
+def test_full_fp64_batch_on_four_workers_equals_one_worker():
+    # Regression: the panel kernel's in-place stores once overtook a
+    # sibling wave's loads of the same rows when the waves drifted apart
+    # under the other workers' kernels (a wrong 32 x 32 block in about one
+    # FP64 batch in four).  Several batches, each bit-identical.
+    probs = fixtures.batch_problems()
+    ref, _ = solve_batch(probs, BATCH_FREQS, precision="fp64", concurrency=1)
+    for rep in range(4):
+        zs, _ = solve_batch(probs, BATCH_FREQS, precision="fp64", concurrency=4)
+        for k, (z, z1) in enumerate(zip(zs, ref)):
+            assert np.array_equal(z.view(np.float64), z1.view(np.float64)), (rep, k)
+    print("batch: four FP64 batches on four workers bit-identical to one worker", flush=True)
+
+
 @pytest.mark.timeout(900)
 def test_full_batch_graded_members_match_the_oracle():
     probs = fixtures.batch_problems()
