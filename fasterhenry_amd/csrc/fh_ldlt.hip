@@ -216,25 +216,57 @@ __device__ __forceinline__ void left_strip_update(float *__restrict__ Re, float 
 // frequencies of the batch per thread: l_mesh is read once.  l_mesh is
 // symmetric, so its row c is read for column c of A (row-contiguous).  Only
 // the lower triangle (r >= c, r < N) is written; no kernel reads the rest.
-// Row y of the grid covers column y (N - y entries) followed by column
-// N - 1 - y (y + 1 entries): N + 1 entries for every y, no idle workgroups.
+// Each thread writes one 16-byte chunk of a column (rows [V q, V q + V),
+// V = 16 / sizeof(T)) of every frequency's matrix: ld is a multiple of 64,
+// so chunks are aligned, and a chunk that meets the diagonal or the end of
+// the column is written element by element.  Row y of the grid covers the
+// chunks of column y followed by those of column N - 1 - y: about N / V + 2
+// chunks for every y, no idle workgroups.
 
 template <typename T>
 __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
                             const double *__restrict__ omegas, int K, int N, int64_t ld,
                             T *__restrict__ re, T *__restrict__ im, int64_t mat_stride) {
+  constexpr int V = 16 / static_cast<int>(sizeof(T));
+  typedef T TV __attribute__((ext_vector_type(V)));
   const int y = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool first = i < N - y;
-  const int c = first ? y : N - 1 - y;
-  const int r = first ? y + i : c + (i - (N - y));
+  const int nch = (N + V - 1) / V;
+  const int n1 = nch - y / V, c2 = N - 1 - y;
+  const bool first = i < n1;
   // the last row of an odd N pairs its column with itself: once
-  if (i > N || r >= N || (!first && c == y)) return;
-  const double l = lmesh[c * loops + r];
-  const int64_t a = c * ld + r;
+  if (!first && c2 <= y) return;
+  const int c = first ? y : c2;
+  const int chunk = first ? y / V + i : c2 / V + (i - n1);
+  if (chunk >= nch) return;
+  const int r0 = chunk * V;
+  double l[V];
+  bool ok[V];
+#pragma unroll
+  for (int e = 0; e < V; ++e) {
+    const int r = r0 + e;
+    ok[e] = r >= c && r < N;
+    l[e] = ok[e] ? lmesh[static_cast<int64_t>(c) * loops + r] : 0.0;
+  }
+  const int64_t a = static_cast<int64_t>(c) * ld + r0;
+  const bool full = r0 >= c && r0 + V <= N;
   for (int f = 0; f < K; ++f) {
-    re[f * mat_stride + a] = T(0);
-    im[f * mat_stride + a] = static_cast<T>(omegas[f] * l);
+    const double w = omegas[f];
+    T *pr = re + f * mat_stride + a, *pi = im + f * mat_stride + a;
+    if (full) {
+      TV vi;
+#pragma unroll
+      for (int e = 0; e < V; ++e) vi[e] = static_cast<T>(w * l[e]);
+      *reinterpret_cast<TV *>(pr) = TV(0);
+      *reinterpret_cast<TV *>(pi) = vi;
+    } else {
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (ok[e]) {
+          pr[e] = T(0);
+          pi[e] = static_cast<T>(w * l[e]);
+        }
+    }
   }
 }
 
@@ -1699,7 +1731,9 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
                     const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
                     const SplitBufs *sb = nullptr) {
   {
-    dim3 grid((L.N + 1 + 255) / 256, (L.N + 1) / 2);
+    // chunks of 16 bytes: a grid row holds at most N / V + 4 of them
+    constexpr int V = 16 / static_cast<int>(sizeof(T));
+    dim3 grid(((L.N + V - 1) / V + 4 + 255) / 256, (L.N + 1) / 2);
     form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, K,
                                              L.N, L.ld, F.re, F.im, L.mat_stride);
     FH_HIP_TRY(hipGetLastError());
