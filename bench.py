@@ -38,9 +38,10 @@ UPDATE_TRAFFIC_BYTES_PER_FREQUENCY = (590.5e6 + 43.5e6) / (41 / 3)
 
 # HBM bytes per pFFT operator product (V = 2) on meander(157): the sum over
 # the product's kernels (spread, line passes, column DFTs, gather, near
-# field) of rocprofv3 FETCH_SIZE (3 860 MB) and WRITE_SIZE (2 366 MB) per
-# product, scaled as above.
-PFFT_TRAFFIC_BYTES = 3860.1e6 + 2366.1e6
+# field) of rocprofv3 FETCH_SIZE (3 593 MB) and WRITE_SIZE (1 382 MB) per
+# product, over the dispatches of the 76 products only (not the set-up's
+# line-pass timing), scaled as above.
+PFFT_TRAFFIC_BYTES = 3593.2e6 + 1381.9e6
 
 
 def cpu_model() -> str:

@@ -320,34 +320,53 @@ __global__ void __launch_bounds__(kLineThreads) line_pass_kernel(
 
 // Axis-2 transforms of short padded length (M2 <= 32) as direct DFTs, one
 // thread per column: forward N2 -> M2 points, inverse M2 -> N2 points.
+// The workgroup's 256 output columns are one contiguous stretch of `out`:
+// the results go through LDS (column stride L_out | 1, odd, so the 8- and
+// 16-byte writes of neighbouring threads fall in different banks) and are
+// stored in order, whole cache lines, instead of one element per thread at
+// a stride of L_out elements.  In place (in == out) needs L_in == L_out:
+// every workgroup then reads and writes only its own stretch.
 template <typename T2>
 __global__ void __launch_bounds__(kThreads) column_dft_kernel(const T2 *in, T2 *out,
                                                                 int64_t cols, int L_in, int L_out, int M,
                                                                 const T2 *__restrict__ tw_global, int inverse) {
   using Re = typename Real<T2>::type;
   __shared__ T2 tw[kDirectMax];
+  extern __shared__ __align__(16) unsigned char smem[];
+  T2 *st = reinterpret_cast<T2 *>(smem);  // [kThreads][L_out | 1]
+  const int P = L_out | 1;
   if (threadIdx.x < M) tw[threadIdx.x] = tw_global[threadIdx.x];
   __syncthreads();
-  const int64_t c = blockIdx.x * static_cast<int64_t>(kThreads) + threadIdx.x;
-  if (c >= cols) return;
-  T2 v[kDirectMax];
+  const int64_t c0 = blockIdx.x * static_cast<int64_t>(kThreads);
+  const int64_t c = c0 + threadIdx.x;
+  if (c < cols) {
+    T2 v[kDirectMax];
 #pragma unroll
-  for (int q = 0; q < kDirectMax; ++q)
-    v[q] = q < L_in ? in[c * L_in + q] : cx(static_cast<Re>(0), static_cast<Re>(0));
-  for (int k = 0; k < L_out; ++k) {
-    T2 acc = cx(static_cast<Re>(0), static_cast<Re>(0));
-    int e = 0;  // (k q) mod M, advanced by k per step
+    for (int q = 0; q < kDirectMax; ++q)
+      v[q] = q < L_in ? in[c * L_in + q] : cx(static_cast<Re>(0), static_cast<Re>(0));
+    for (int k = 0; k < L_out; ++k) {
+      T2 acc = cx(static_cast<Re>(0), static_cast<Re>(0));
+      int e = 0;  // (k q) mod M, advanced by k per step
 #pragma unroll
-    for (int q = 0; q < kDirectMax; ++q) {
-      if (q < L_in) {
-        T2 w = tw[e];
-        if (inverse) w.y = -w.y;
-        acc = cadd(acc, cmul(v[q], w));
-        e += k;
-        if (e >= M) e -= M;
+      for (int q = 0; q < kDirectMax; ++q) {
+        if (q < L_in) {
+          T2 w = tw[e];
+          if (inverse) w.y = -w.y;
+          acc = cadd(acc, cmul(v[q], w));
+          e += k;
+          if (e >= M) e -= M;
+        }
       }
+      st[threadIdx.x * P + k] = acc;
     }
-    out[c * L_out + k] = acc;
+  }
+  __syncthreads();
+  const int ncol = static_cast<int>(min(static_cast<int64_t>(kThreads), cols - c0));
+  const int total = ncol * L_out;
+  T2 *dst = out + c0 * L_out;
+  for (int e = threadIdx.x; e < total; e += kThreads) {
+    const int cc = e / L_out, k = e - cc * L_out;
+    dst[e] = st[cc * P + k];
   }
 }
 
@@ -405,8 +424,14 @@ struct ConvolverImpl {
     if (lines <= 0) return FH_OK;
     const int Mx = plan[axis].M;
     if (axis == 2 && mode != 2 && Mx <= kDirectMax && B == 1) {
-      column_dft_kernel<T2><<<blocks_for(lines), kThreads, 0, stream>>>(in, out, lines, L_in, L_out, Mx,
-                                                                         tw + tw_off[axis], mode == 1 ? 1 : 0);
+      // in place only column for column (each workgroup's stretch is its own)
+      if (in == out && L_in != L_out)
+        return set_error(FH_E_PFFT, "axis-2 direct DFT in place with unequal lengths", 0, 0, 0.0, 2);
+      const int64_t blocks = (lines + kThreads - 1) / kThreads;
+      if (blocks > 0x7fffffff) return set_error(FH_E_PFFT, "pFFT grid too large for the axis-2 DFT", 0, 0, 0.0, 2);
+      const size_t lds = static_cast<size_t>(kThreads) * (L_out | 1) * sizeof(T2);
+      column_dft_kernel<T2><<<static_cast<unsigned>(blocks), kThreads, lds, stream>>>(
+          in, out, lines, L_in, L_out, Mx, tw + tw_off[axis], mode == 1 ? 1 : 0);
       FH_HIP_TRY(hipGetLastError());
       return FH_OK;
     }
@@ -583,7 +608,9 @@ fh_status PrunedConvolver::init(const int64_t N[3], const int64_t M[3], double h
   for (const void *k : {reinterpret_cast<const void *>(&line_pass_kernel<double2, true>),
                         reinterpret_cast<const void *>(&line_pass_kernel<double2, false>),
                         reinterpret_cast<const void *>(&line_pass_kernel<float2, true>),
-                        reinterpret_cast<const void *>(&line_pass_kernel<float2, false>)})
+                        reinterpret_cast<const void *>(&line_pass_kernel<float2, false>),
+                        reinterpret_cast<const void *>(&column_dft_kernel<double2>),
+                        reinterpret_cast<const void *>(&column_dft_kernel<float2>)})
     FH_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBudget));
   // spectrum: FP64 line passes over the full padded grid, then its even half
   {
