@@ -549,6 +549,82 @@ __device__ __forceinline__ void tile_mma(TileAcc<T> &a, const T (*pr)[kT], const
   }
 }
 
+// Panel product of the refined form on the bf16 matrix cores, at the
+// precision of the outer updates: W[r][c] = sum_k A21[r][k] Linv[c][k] over
+// the block's k < w, each FP32 operand split as x = hi + lo and each real
+// product formed as hi hi + hi lo + lo hi (twelve 16x16x32 instructions per
+// complex 16 x 16 x 32 block).  Operands straight from memory into the
+// operand order (A: lane l holds P[c = l % 16][k = 8 (l / 16) + e], B: lane l
+// holds Q[k = 8 (l / 16) + e][r = l % 16]); P = Linv^T is zero outside the
+// block, Q is masked to k < w and r < N.  The accumulators have the layout
+// of the FP32 16x16x4 instruction's, so the stores that follow are shared.
+__device__ __forceinline__ void panel_product_bf16(TileAcc<float> &acc, const float *__restrict__ lt_re,
+                                                   const float *__restrict__ lt_im, int64_t lbase,
+                                                   const float *__restrict__ Re,
+                                                   const float *__restrict__ Im, int64_t ld, int k0,
+                                                   int w, int r0, int N, int wi, int wj, int lane) {
+  const int kq = lane >> 4, x16 = lane & 15;
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+      acc.re[ib][jb] = f4{0, 0, 0, 0};
+      acc.im[ib][jb] = f4{0, 0, 0, 0};
+    }
+#pragma unroll
+  for (int kc = 0; kc < kB; kc += 32) {
+    if (kc >= w) break;
+    bf16x8 p[2][4], q[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = kc + 8 * kq + e;
+        const float ar = lt_re[lbase + k * kB + wi + 16 * b + x16];
+        const float ai = lt_im[lbase + k * kB + wi + 16 * b + x16];
+        const int rr = r0 + wj + 16 * b + x16;
+        const int64_t kcl = min(k, w - 1), rcl = min(rr, N - 1);
+        const float qa = Re[(k0 + kcl) * ld + rcl], qc = Im[(k0 + kcl) * ld + rcl];
+        const bool ok = k < w && rr < N;
+        __bf16 h, l;
+        split_bf16(ar, h, l);
+        p[b][0][e] = h;
+        p[b][1][e] = l;
+        split_bf16(ai, h, l);
+        p[b][2][e] = h;
+        p[b][3][e] = l;
+        split_bf16(ok ? qa : 0.f, h, l);
+        q[b][0][e] = h;
+        q[b][1][e] = l;
+        split_bf16(ok ? qc : 0.f, h, l);
+        q[b][2][e] = h;
+        q[b][3][e] = l;
+      }
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      const bf16x8 nih = neg8(p[ib][2]), nil = neg8(p[ib][3]);
+#pragma unroll
+      for (int jb = 0; jb < 2; ++jb) {
+        f4 cr = acc.re[ib][jb], ci = acc.im[ib][jb];
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][1], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[jb][0], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][2], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, q[jb][3], cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, q[jb][2], cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][0], q[jb][3], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][1], q[jb][2], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[jb][0], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][2], q[jb][1], ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_16x16x32_bf16(p[ib][3], q[jb][0], ci, 0, 0, 0);
+        acc.re[ib][jb] = cr;
+        acc.im[ib][jb] = ci;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2. Panel at k0 (width w, block blk): rows r in [k0 + w, N).
 // W[r][c] = sum_k A21[r][k] Linv[c][k] (stored at W column wcol + c);
@@ -569,7 +645,7 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
                                                     int64_t split_stride, int G, int write_w,
                                                     const bf16x8 *__restrict__ lsrc,
                                                     const bf16x8 *__restrict__ wsrc, int kc_beg,
-                                                    int kc_end) {
+                                                    int kc_end, int bf16_product) {
   const int f = blockIdx.y;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   T *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
@@ -628,26 +704,35 @@ __global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__res
       dvi[ib][v] = dinv_im[dbase + c];
     }
   TileAcc<T> acc;
-  tile_zero(acc);
-  Chunk cur, nxt;
-  load(0, cur);
+  bool done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (bf16_product) {
+      panel_product_bf16(acc, linvt_re, linvt_im, lbase, Re, Im, ld, k0, w, r0, N, wi, wj, lane);
+      done = true;
+    }
+  }
+  if (!done) {
+    tile_zero(acc);
+    Chunk cur, nxt;
+    load(0, cur);
 #pragma unroll
-  for (int c = 0; c < kB / 16; ++c) {
-    const int kc = 16 * c;
-    if (kc >= w) break;
-    if (kc + 16 < w) load(kc + 16, nxt);
+    for (int c = 0; c < kB / 16; ++c) {
+      const int kc = 16 * c;
+      if (kc >= w) break;
+      if (kc + 16 < w) load(kc + 16, nxt);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int ib = 0; ib < 2; ++ib)
+        for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
-        for (int jb = 0; jb < 2; ++jb) {
-          acc.re[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.br[q][jb], acc.re[ib][jb]);
-          acc.re[ib][jb] = Mfma<T>::run(-cur.ai[q][ib], cur.bi[q][jb], acc.re[ib][jb]);
-          acc.im[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.bi[q][jb], acc.im[ib][jb]);
-          acc.im[ib][jb] = Mfma<T>::run(cur.ai[q][ib], cur.br[q][jb], acc.im[ib][jb]);
-        }
-    cur = nxt;
+          for (int jb = 0; jb < 2; ++jb) {
+            acc.re[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.br[q][jb], acc.re[ib][jb]);
+            acc.re[ib][jb] = Mfma<T>::run(-cur.ai[q][ib], cur.bi[q][jb], acc.re[ib][jb]);
+            acc.im[ib][jb] = Mfma<T>::run(cur.ar[q][ib], cur.bi[q][jb], acc.im[ib][jb]);
+            acc.im[ib][jb] = Mfma<T>::run(cur.ai[q][ib], cur.br[q][jb], acc.im[ib][jb]);
+          }
+      cur = nxt;
+    }
   }
   // The results go in place: waves 0 and 2 (1 and 3) read the same 32 rows
   // over all 64 columns and each writes half of those columns.  Every
@@ -1148,6 +1233,21 @@ __global__ void dscale_kernel(const T *__restrict__ re, const T *__restrict__ im
   const Z2 y = zmul(Z2{in[o], in[o + m]}, inv);
   out[o] = y.re;
   out[o + m] = y.im;
+}
+
+// out[v] = the port row q of frequency f's factor over the m interior
+// columns, v = f p + q: L[m + q][i] for i < m ({re}, {im} halves).
+template <typename T>
+__global__ void port_rows_kernel(const T *__restrict__ re, const T *__restrict__ im, int64_t ld,
+                                 int64_t mat_stride, int m, int p, int V, double *__restrict__ out) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(V) * m) return;
+  const int v = static_cast<int>(t / m), i = static_cast<int>(t % m);
+  const int f = v / p, q = v % p;
+  const int64_t a = f * mat_stride + static_cast<int64_t>(i) * ld + m + q;
+  const int64_t o = static_cast<int64_t>(2 * v) * m + i;
+  out[o] = static_cast<double>(re[a]);
+  out[o + m] = static_cast<double>(im[a]);
 }
 
 // Backward, diagonal part of the super-block at B0: solves L_BB^T x = z_B
@@ -1776,7 +1876,7 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
           F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk, L.N, F.wre, F.wim, L.w_stride,
           k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im, emit ? sb->l : nullptr,
           emit ? sb->w : nullptr, sb ? sb->stride : 0, sb ? sb->G : 0, sb ? 0 : 1,
-          sb ? sb->l : nullptr, sb ? sb->w : nullptr, ikc_beg, ikc_end);
+          sb ? sb->l : nullptr, sb ? sb->w : nullptr, ikc_beg, ikc_end, sb ? 1 : 0);
       FH_HIP_TRY(hipGetLastError());
       if (sb) continue;
       if (k0 + w < KE)
@@ -1791,6 +1891,25 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
     }
     FH_HIP_TRY(hipGetLastError());
   }
+  return FH_OK;
+}
+
+// The backward half of a solve: vec <- L^-T work (work: the forward
+// substitution already scaled by D^-1).
+template <typename T>
+fh_status factor_solve_bwd(const Layout &L, const Factor<T> &F, int V, double *vec, double *work,
+                           hipStream_t stream) {
+  const int m = L.m;
+  const int nsb = (m + kSB - 1) / kSB;
+  for (int sb = nsb - 1; sb >= 0; --sb) {
+    const int B0 = sb * kSB, bw = std::min(kSB, m - B0);
+    bwd_diag_kernel<T><<<V, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw,
+                                              L.nblk, F.lt_re, F.lt_im, work, vec);
+    if (B0 > 0)
+      bwd_update_kernel<T><<<dim3((B0 + kB - 1) / kB, V), 256, 0, stream>>>(
+          F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw, work, vec);
+  }
+  FH_HIP_TRY(hipGetLastError());
   return FH_OK;
 }
 
@@ -1817,16 +1936,25 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
   dscale_kernel<T><<<static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256), 256, 0, stream>>>(
       F.re, F.im, L.ld, L.mat_stride, m, L.p, V, vec, work);
   FH_HIP_TRY(hipGetLastError());
-  for (int sb = nsb - 1; sb >= 0; --sb) {
-    const int B0 = sb * kSB, bw = std::min(kSB, m - B0);
-    bwd_diag_kernel<T><<<V, 256, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw,
-                                              L.nblk, F.lt_re, F.lt_im, work, vec);
-    if (B0 > 0)
-      bwd_update_kernel<T><<<dim3((B0 + kB - 1) / kB, V), 256, 0, stream>>>(
-          F.re, F.im, L.ld, L.mat_stride, m, L.p, B0, bw, work, vec);
-  }
+  return factor_solve_bwd<T>(L, F, V, vec, work, stream);
+}
+
+// v <- Zee^-1 b for the V port columns b of Zee's border (x = 0, r = b: the
+// first step of a refinement), without the forward substitution.  The
+// bordered factorization holds the port rows L_p of the factor, and with
+// Z_pe = L_p D L^T (rows m + q, columns < m) the forward substitution of
+// b = Z_ep is L^-1 b = D L_p^T: after D^-1 it is L_p^T itself, read from the
+// factor, and the backward substitution finishes the solve.  (In floating
+// point the two differ by the factor's own rounding; the refinement that
+// follows corrects either.)
+template <typename T>
+fh_status factor_solve_border(const Layout &L, const Factor<T> &F, int V, double *vec, double *work,
+                              hipStream_t stream) {
+  const size_t n = static_cast<size_t>(V) * L.m;
+  port_rows_kernel<T><<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
+      F.re, F.im, L.ld, L.mat_stride, L.m, L.p, V, work);
   FH_HIP_TRY(hipGetLastError());
-  return FH_OK;
+  return factor_solve_bwd<T>(L, F, V, vec, work, stream);
 }
 
 }  // namespace
@@ -2184,8 +2312,13 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     refine_state_init_kernel<<<kb, 256, 0, stream>>>(K, d_state);
     if ((s2 = residual())) return s2;
     for (int it = 1; it <= kGraphSteps; ++it) {
-      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
-      if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
+      if (it == 1) {
+        // X = 0, R = b: the first correction from the factor's port rows
+        if ((s2 = factor_solve_border<float>(L, F, V, DX, P, stream))) return s2;
+      } else {
+        axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
+        if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
+      }
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
       if ((s2 = residual())) return s2;
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
