@@ -288,9 +288,9 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // outside the block) and D^-1 for block blk of frequency f.
 //
 // One workgroup per frequency; lane r of every wave holds row r, and wave g
-// of the eight holds the columns 8g .. 8g + 7 in registers (8 slots), so a
-// column step costs each lane 8 complex multiply-adds and one barrier, with
-// two waves per SIMD to cover each other's dependency stalls.  Pivot columns
+// of the W waves holds the columns S g .. S g + S - 1 in registers (S = 64 / W
+// slots), so a column step costs each lane S complex multiply-adds and one
+// barrier.  Pivot columns
 // reach the other waves through a two-entry LDS ring.  Linv is built in the
 // same pass: the row operations of the elimination (row r -= l_rc row c) are
 // applied to an identity X, which becomes L11^-1.  Slot j holds A[r][j]
@@ -300,7 +300,7 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // X[r][j] -= l_r X[c][j] with X[c][j] read from lane c; for j = c the slot
 // becomes X[r][c] = delta_rc - l_r.
 //
-// Column c = 8q + t is held by wave q, slot t.  The step loop is unrolled
+// Column c = S q + t is held by wave q, slot t.  The step loop is unrolled
 // over t, so in every wave each slot's role at a step is fixed by one
 // wave-uniform branch: waves below q hold only columns left of c (all X
 // updates), waves above q only columns right of it (all A updates), and in
@@ -313,10 +313,20 @@ __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t
 // FP32 for the FP32 one (whose products elsewhere carry split-bf16 rounding
 // far above FP32's, and which refinement in FP64 corrects).
 
-// Waves of the diagonal-block kernel (two per SIMD) and the columns each
-// holds in registers.
-constexpr int kDiagWaves = 8;
-constexpr int kDiagSlots = kB / kDiagWaves;
+// Waves of the diagonal-block kernel.  FP64: eight waves of eight column slots (two per SIMD).  FP32 (refined
+// form): four waves of sixteen, one per SIMD at 124 registers, so that the
+// workgroup fits beside a trailing-update wave of another frequency group
+// (344 of a SIMD's 512 registers) instead of waiting for the update's
+// workgroups to drain: 109-117 -> 92 us per launch beside the updates
+// (44 -> 50 us alone).
+template <typename T>
+struct DiagShape {
+  static constexpr int waves = 8;
+};
+template <>
+struct DiagShape<float> {
+  static constexpr int waves = 4;
+};
 
 template <typename C>
 struct Zc {
@@ -364,7 +374,7 @@ __device__ __forceinline__ void diag_a(C &mr, C &mi, Zc<C> l, C br, C bi) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ re, T *__restrict__ im,
+__global__ void __launch_bounds__(64 * DiagShape<T>::waves) diag_kernel(T *__restrict__ re, T *__restrict__ im,
                                                    int64_t ld, int64_t mat_stride, int k0, int w,
                                                    int blk, int nblk, T *__restrict__ linvt_re,
                                                    T *__restrict__ linvt_im, T *__restrict__ dinv_re,
@@ -375,7 +385,7 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
                                                    int64_t split_stride, int G, int kc_beg,
                                                    int kc_end) {
   using C = T;
-  constexpr int S = kDiagSlots;
+  constexpr int W = DiagShape<T>::waves, S = kB / W;
   const int f = blockIdx.x;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -412,7 +422,7 @@ __global__ void __launch_bounds__(64 * kDiagWaves) diag_kernel(T *__restrict__ r
   }
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
   __syncthreads();
-  for (int q = 0; q < kDiagWaves; ++q) {
+  for (int q = 0; q < W; ++q) {
 #pragma unroll
     for (int t = 0; t < S; ++t) {
       const int c = S * q + t;
@@ -631,8 +641,11 @@ __device__ __forceinline__ void panel_product_bf16(TileAcc<float> &acc, const fl
 // L21[r][c] = W[r][c] Dinv[c] (stored in place).  D[i = c][j = r] with
 // P[c][k] = linvt[k][c] and Q[k][r] = A21[r][k].
 
+// At most 168 registers (three waves per SIMD), so that a panel workgroup
+// fits beside a trailing-update wave of another frequency group (which
+// holds 344 of a SIMD's 512 registers).
 template <typename T>
-__global__ void __launch_bounds__(256) panel_kernel(T *__restrict__ re, T *__restrict__ im,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) panel_kernel(T *__restrict__ re, T *__restrict__ im,
                                                     int64_t ld, int64_t mat_stride, int k0, int w,
                                                     int blk, int nblk, int N, T *__restrict__ wre,
                                                     T *__restrict__ wim, int64_t w_stride, int wcol,
@@ -1861,7 +1874,7 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
       // blocks of this outer block, from their split copies, applied by the
       // diagonal-block and panel kernels to their own rows
       const int ikc_beg = sb && k0 > K0 ? K0 / 32 : 0, ikc_end = sb && k0 > K0 ? k0 / 32 : 0;
-      diag_kernel<T><<<K, 64 * kDiagWaves, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
+      diag_kernel<T><<<K, 64 * DiagShape<T>::waves, 0, stream>>>(F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk,
                                             F.lt_re, F.lt_im, F.di_re, F.di_im, d_singular,
                                             sb ? sb->l : nullptr, sb ? sb->w : nullptr,
                                             sb ? sb->stride : 0, sb ? sb->G : 0, ikc_beg, ikc_end);
