@@ -1462,6 +1462,18 @@ __global__ void axpy_copy_kernel(int64_t n, double *__restrict__ x, const double
   if (next) next[t] = r[t];
 }
 
+// x += dx for the vectors of the frequencies still refining (active[f]):
+// the steps after the first ones run for a whole group of frequencies, and a
+// frequency already accepted keeps its x (and Z) whatever its neighbours in
+// the group need, so its result does not depend on how the sweep is grouped.
+__global__ void active_axpy_kernel(int m, int p, int64_t n, double *__restrict__ x,
+                                   const double *__restrict__ dx, const int *__restrict__ active) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  const int f = static_cast<int>(t / (2 * static_cast<int64_t>(m))) / p;
+  if (active[f]) x[t] += dx[t];
+}
+
 // Z[f][q1][q2] = Zpp - sum_i b^{q1}_i x^{q2}_i - sum_i x^{q1}_i r^{q2}_i and
 // the relative size of the last (first-order) term; one block per entry,
 // fixed-order reduction.
@@ -1469,9 +1481,11 @@ __global__ void __launch_bounds__(256) functional_kernel(
     int m, int p, const double *__restrict__ omegas, const double *__restrict__ rep,
     const double *__restrict__ rpp, const double *__restrict__ lmesh, int64_t lds,
     const double *__restrict__ X, const double *__restrict__ Rv, double *__restrict__ z,
-    double *__restrict__ metric) {
+    double *__restrict__ metric, const int *__restrict__ active) {
   const int e = blockIdx.x;
   const int f = e / (p * p), q1 = (e / p) % p, q2 = e % p;
+  // (a frequency already accepted keeps its Z; the whole block leaves)
+  if (active && !active[f]) return;
   const double w = omegas[f];
   const int v1 = f * p + q1, v2 = f * p + q2;
   const double *x1r = X + static_cast<int64_t>(2 * v1) * m, *x1i = x1r + m;
@@ -2032,7 +2046,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
                        4 * static_cast<size_t>(V) * hld * sizeof(double) +  // h (2 passes)
                        2 * static_cast<size_t>(V) * hld * sizeof(double) +  // y
                        6 * static_cast<size_t>(K) * sizeof(double) +        // device refinement state
-                       2 * static_cast<size_t>(K) * sizeof(int) + 256;      // sing, reset
+                       3 * static_cast<size_t>(K) * sizeof(int) + 256;      // sing, reset, active
   const size_t extra = (5 + static_cast<size_t>(lx_slices) + hld) * vbytes + small;
   const size_t sbytes = static_cast<size_t>(split_blocks(L)) * 4096 * K;
   if ((st = ws.reserve(fbytes + extra + 2 * sbytes + 512))) return st;
@@ -2062,12 +2076,13 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
   double *d_state = d_y + 2 * static_cast<size_t>(V) * hld;
   int *d_sing = reinterpret_cast<int *>(d_state + 6 * static_cast<size_t>(K));
   int *d_reset = d_sing + K;
+  int *d_active = d_reset + K;
   const unsigned vec_blocks = static_cast<unsigned>((n + 255) / 256);
   const unsigned res_blocks = static_cast<unsigned>((static_cast<size_t>(V) * m + 255) / 256);
   const int64_t lds = L.N;
   std::vector<double> h_metric(static_cast<size_t>(K) * p * p), h_nrm(2 * static_cast<size_t>(V));
   std::vector<double> h_z(2 * static_cast<size_t>(K) * p * p);
-  std::vector<int> h_sing(K, 0), h_reset(K, 0);
+  std::vector<int> h_sing(K, 0), h_reset(K, 0), h_active(K, 1);
 
   // P = L X, then R = b - Zee X.
   auto residual = [&]() -> fh_status {
@@ -2223,7 +2238,17 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
       prev_dx.assign(K, 0.0);
     }
     fh_status s2;
+    // frequencies that leave this stage early (their contraction is too
+    // slow): each frequency's steps depend on its own history only
+    std::vector<char> left(K, 0);
     for (int it = first_it; it <= max_steps; ++it) {
+      int running = 0;
+      for (int f = 0; f < K; ++f) {
+        h_active[f] = out.converged[f] || left[f] ? 0 : 1;
+        running += h_active[f];
+      }
+      if (running == 0) break;
+      FH_HIP_TRY(hipMemcpyAsync(d_active, h_active.data(), K * sizeof(int), hipMemcpyHostToDevice, stream));
       if (gmres) {
         int inner = 0;
         if ((s2 = gmres_correction(inner))) return s2;
@@ -2231,10 +2256,10 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
         axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
         if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
       }
-      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
+      active_axpy_kernel<<<vec_blocks, 256, 0, stream>>>(m, p, static_cast<int64_t>(n), X, DX, d_active);
       if ((s2 = residual())) return s2;
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
-                                                        in.d_lmesh, lds, X, R, d_z, metric);
+                                                        in.d_lmesh, lds, X, R, d_z, metric, d_active);
       norms_kernel<<<V, 256, 0, stream>>>(m, DX, R, d_nrm);
       FH_HIP_TRY(hipGetLastError());
       FH_HIP_TRY(hipMemcpyAsync(h_metric.data(), metric, h_metric.size() * sizeof(double),
@@ -2246,7 +2271,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
       FH_HIP_TRY(hipStreamSynchronize(stream));
       bool all = true;
       for (int f = 0; f < K; ++f) {
-        if (out.converged[f]) continue;
+        if (!h_active[f]) continue;
         double worst = 0.0;
         for (int q = 0; q < p * p; ++q)
           worst = std::max(worst, h_metric[static_cast<size_t>(f) * p * p + q]);
@@ -2293,13 +2318,10 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
       }
       if (all) break;
       // A contraction slower than 1/4 per step will not reach the tolerance
-      // within the budget: stop early and move on to the next stage.
-      if (it >= 3) {
-        bool slow = false;
+      // within the budget: such a frequency leaves early for the next stage.
+      if (it >= 3)
         for (int f = 0; f < K; ++f)
-          if (!out.converged[f] && !(out.refine_change[f] < 1e-6)) slow = true;
-        if (slow) break;
-      }
+          if (h_active[f] && !out.converged[f] && !(out.refine_change[f] < 1e-6)) left[f] = 1;
     }
     return FH_OK;
   };
@@ -2335,7 +2357,7 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
       if ((s2 = residual())) return s2;
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
-                                                        in.d_lmesh, lds, X, R, d_z, metric);
+                                                        in.d_lmesh, lds, X, R, d_z, metric, nullptr);
       norms_kernel<<<V, 256, 0, stream>>>(m, DX, R, d_nrm);
       refine_accept_kernel<<<kb, 256, 0, stream>>>(K, p, it, metric, d_nrm, d_z, d_sing, kZTol, kTol,
                                                    d_state);
