@@ -1248,21 +1248,6 @@ __global__ void dscale_kernel(const T *__restrict__ re, const T *__restrict__ im
   out[o + m] = y.im;
 }
 
-// out[v] = the port row q of frequency f's factor over the m interior
-// columns, v = f p + q: L[m + q][i] for i < m ({re}, {im} halves).
-template <typename T>
-__global__ void port_rows_kernel(const T *__restrict__ re, const T *__restrict__ im, int64_t ld,
-                                 int64_t mat_stride, int m, int p, int V, double *__restrict__ out) {
-  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<int64_t>(V) * m) return;
-  const int v = static_cast<int>(t / m), i = static_cast<int>(t % m);
-  const int f = v / p, q = v % p;
-  const int64_t a = f * mat_stride + static_cast<int64_t>(i) * ld + m + q;
-  const int64_t o = static_cast<int64_t>(2 * v) * m + i;
-  out[o] = static_cast<double>(re[a]);
-  out[o + m] = static_cast<double>(im[a]);
-}
-
 // Backward, diagonal part of the super-block at B0: solves L_BB^T x = z_B
 // for z_B = work[B0, B0 + bw) (sub-blocks last to first) and writes x to out.
 template <typename T>
@@ -1966,23 +1951,6 @@ fh_status factor_solve(const Layout &L, const Factor<T> &F, int V, double *vec, 
   return factor_solve_bwd<T>(L, F, V, vec, work, stream);
 }
 
-// v <- Zee^-1 b for the V port columns b of Zee's border (x = 0, r = b: the
-// first step of a refinement), without the forward substitution.  The
-// bordered factorization holds the port rows L_p of the factor, and with
-// Z_pe = L_p D L^T (rows m + q, columns < m) the forward substitution of
-// b = Z_ep is L^-1 b = D L_p^T: after D^-1 it is L_p^T itself, read from the
-// factor, and the backward substitution finishes the solve.  (In floating
-// point the two differ by the factor's own rounding; the refinement that
-// follows corrects either.)
-template <typename T>
-fh_status factor_solve_border(const Layout &L, const Factor<T> &F, int V, double *vec, double *work,
-                              hipStream_t stream) {
-  const size_t n = static_cast<size_t>(V) * L.m;
-  port_rows_kernel<T><<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
-      F.re, F.im, L.ld, L.mat_stride, L.m, L.p, V, work);
-  FH_HIP_TRY(hipGetLastError());
-  return factor_solve_bwd<T>(L, F, V, vec, work, stream);
-}
 
 }  // namespace
 
@@ -2347,13 +2315,8 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     refine_state_init_kernel<<<kb, 256, 0, stream>>>(K, d_state);
     if ((s2 = residual())) return s2;
     for (int it = 1; it <= kGraphSteps; ++it) {
-      if (it == 1) {
-        // X = 0, R = b: the first correction from the factor's port rows
-        if ((s2 = factor_solve_border<float>(L, F, V, DX, P, stream))) return s2;
-      } else {
-        axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
-        if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
-      }
+      axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
+      if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, DX, nullptr, nullptr);
       if ((s2 = residual())) return s2;
       functional_kernel<<<K * p * p, 256, 0, stream>>>(m, p, d_omegas, in.d_rep, in.d_rpp,
