@@ -1439,6 +1439,21 @@ __global__ void residual_kernel(int m, int p, int V, const double *__restrict__ 
   Rv[static_cast<int64_t>(2 * v + 1) * m + i] = b_im - si - w * pr;
 }
 
+// R = b for X = 0: what residual_kernel gives there (L X = +0, so
+// R_re = b_re - 0 + w 0 = b_re + 0 and R_im = b_im - 0 - w 0 = b_im), without
+// the product L X.
+__global__ void border_rhs_kernel(int m, int p, int V, const double *__restrict__ omegas,
+                                  const double *__restrict__ rep, const double *__restrict__ lmesh,
+                                  int64_t lds, double *__restrict__ Rv) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<int64_t>(V) * m) return;
+  const int v = static_cast<int>(t / m), i = static_cast<int>(t % m);
+  const int f = v / p, q = v % p;
+  const double w = omegas[f];
+  Rv[static_cast<int64_t>(2 * v) * m + i] = rep[static_cast<int64_t>(q) * m + i] + 0.0;
+  Rv[static_cast<int64_t>(2 * v + 1) * m + i] = w * lmesh[static_cast<int64_t>(m + q) * lds + i];
+}
+
 __global__ void axpy_copy_kernel(int64_t n, double *__restrict__ x, const double *__restrict__ dx,
                                  double *__restrict__ next, const double *__restrict__ r) {
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -2313,7 +2328,9 @@ fh_status ldlt_refined_batch(const DenseSolveInputs &in, const double *d_omegas,
     if ((s2 = factorize<float>(in, L, d_omegas, K, F, d_sing, stream, pr, &sb))) return s2;
     fill_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), X, 0.0);
     refine_state_init_kernel<<<kb, 256, 0, stream>>>(K, d_state);
-    if ((s2 = residual())) return s2;
+    // the residual of X = 0 is b itself
+    border_rhs_kernel<<<res_blocks, 256, 0, stream>>>(m, p, V, d_omegas, in.d_rep, in.d_lmesh, lds, R);
+    FH_HIP_TRY(hipGetLastError());
     for (int it = 1; it <= kGraphSteps; ++it) {
       axpy_copy_kernel<<<vec_blocks, 256, 0, stream>>>(static_cast<int64_t>(n), nullptr, nullptr, DX, R);
       if ((s2 = factor_solve<float>(L, F, V, DX, P, stream))) return s2;
