@@ -938,25 +938,33 @@ __global__ void __launch_bounds__(256) update_kernel(T *__restrict__ re, T *__re
 
 
 // Tiles of the left-looking update: 128 x 128, rows r in [J, N), columns s
-// in [J, J + nbj); tile (tr, tc) with tc < ctiles and tr >= tc.  Listed row
-// by row, so the tiles of one row (sharing the W operand) are adjacent, and
-// dealt to the XCDs in contiguous stretches (the hardware assigns
-// workgroup b to XCD b % 8).
-__device__ __forceinline__ bool left_tile(int b, int rtiles, int ctiles, int &tr, int &tc) {
+// in [J, J + nbj); tile (tr, tc) with tc < ctiles and tr >= tc, listed row
+// by row, so the tiles of one row (sharing the W operand) are adjacent.  The
+// launch's work units are the (frequency, tile) pairs, frequency-major, dealt
+// to the XCDs in contiguous eighths (the hardware assigns workgroup b to XCD
+// b % 8): every XCD gets the same number of units to within one, and the
+// tiles of one frequency (sharing its L operand) meet in one XCD's L2.
+// Dealt by tile alone, a launch whose tile count is 8 k - 5 left one XCD five
+// tiles short for every frequency, idle for part of the launch.
+__device__ __forceinline__ bool left_unit(int b, int rtiles, int ctiles, int K, int &f, int &tr,
+                                          int &tc) {
   const int tri = ctiles * (ctiles - 1) / 2;
   const int T = tri + (rtiles - (ctiles - 1)) * ctiles;
-  const int per = (T + 7) / 8;
-  const int t = (b % 8) * per + b / 8;
-  if (t >= T) return false;
+  const int units = T * K;
+  const int per = (units + 7) / 8;
+  const int u = (b % 8) * per + b / 8;
+  if (u >= units) return false;
+  f = u / T;
+  const int t = u - f * T;
   if (t < tri) {
     tr = static_cast<int>((__builtin_sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while (tr * (tr + 1) / 2 > t) --tr;
     while ((tr + 1) * (tr + 2) / 2 <= t) ++tr;
     tc = t - tr * (tr + 1) / 2;
   } else {
-    const int u = t - tri;
-    tr = ctiles - 1 + u / ctiles;
-    tc = u % ctiles;
+    const int v = t - tri;
+    tr = ctiles - 1 + v / ctiles;
+    tc = v % ctiles;
   }
   return true;
 }
@@ -965,10 +973,9 @@ __global__ void __launch_bounds__(256) update_left_kernel(
     float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int J,
     int nbj, int N, int G, int rtiles, int ctiles, int kc_beg, int kc_end,
     const bf16x8 *__restrict__ lsplit, const bf16x8 *__restrict__ wsplit, int64_t split_stride,
-    int narrow) {
-  const int f = blockIdx.y;
-  int tr, tc;
-  if (!left_tile(blockIdx.x, rtiles, ctiles, tr, tc)) return;
+    int narrow, int K) {
+  int f, tr, tc;
+  if (!left_unit(blockIdx.x, rtiles, ctiles, K, f, tr, tc)) return;
   // The wave index is uniform across the wave: keeping it (and every
   // address derived from it) in scalar registers leaves the vector
   // registers to the operands and accumulators.
@@ -1837,9 +1844,12 @@ fh_status launch_left_update(const Layout &L, const Factor<float> &F, const Spli
   const int ctiles = narrow ? 1 : (nbj + 127) / 128;
   const int T = ctiles * (ctiles - 1) / 2 + (rtiles - (ctiles - 1)) * ctiles;
   if (prof) prof->begin(stream);
-  update_left_kernel<<<dim3(((T + 7) / 8) * 8, K), 256, 0, stream>>>(
+  const int64_t units = static_cast<int64_t>(T) * K;
+  if (units > 0x7fffff00)
+    return set_error(FH_E_INVALID_ARGUMENT, "left-looking update: too many tiles for one launch");
+  update_left_kernel<<<static_cast<unsigned>((units + 7) / 8 * 8), 256, 0, stream>>>(
       F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, kc_beg, kc_end, sb.l,
-      sb.w, sb.stride, narrow);
+      sb.w, sb.stride, narrow, K);
   if (prof) {
     double entries = 0.0;
     for (int s = J; s < J + nbj; ++s) entries += static_cast<double>(L.N - s);
