@@ -658,13 +658,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) p
                                                     int64_t split_stride, int G, int write_w,
                                                     const bf16x8 *__restrict__ lsrc,
                                                     const bf16x8 *__restrict__ wsrc, int kc_beg,
-                                                    int kc_end, int bf16_product) {
-  const int f = blockIdx.y;
+                                                    int kc_end, int bf16_product, int ptiles,
+                                                    int K) {
+  // (frequency, row tile) units, frequency-major, dealt to the XCDs in
+  // contiguous eighths: a frequency's tiles (sharing its L11^-1 and the
+  // strip operands) meet in one L2, and the XCDs get equal shares
+  const int units = ptiles * K, per = (units + 7) / 8;
+  const int u = static_cast<int>(blockIdx.x % 8) * per + static_cast<int>(blockIdx.x / 8);
+  if (u >= units) return;
+  const int f = u / ptiles;
   T *Re = re + f * mat_stride, *Im = im + f * mat_stride;
   T *Wr = wre + f * w_stride, *Wi = wim + f * w_stride;
   const int64_t lbase = (static_cast<int64_t>(f) * nblk + blk) * kB * kB;
   const int64_t dbase = (static_cast<int64_t>(f) * nblk + blk) * kB;
-  const int r0 = k0 + w + blockIdx.x * kT;
+  const int r0 = k0 + w + (u - f * ptiles) * kT;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if constexpr (std::is_same<T, float>::value) {
     // left-looking path: these panel rows' update by the columns of the
@@ -1909,11 +1916,12 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
       // on the left-looking path the panel also writes the split copies of
       // its W and L (needed when columns remain), and no FP32 W
       const bool emit = sb && k0 + w < L.m;
-      panel_kernel<T><<<dim3(ptiles, K), 256, 0, stream>>>(
+      const unsigned pblocks = static_cast<unsigned>((static_cast<int64_t>(ptiles) * K + 7) / 8 * 8);
+      panel_kernel<T><<<pblocks, 256, 0, stream>>>(
           F.re, F.im, L.ld, L.mat_stride, k0, w, blk, L.nblk, L.N, F.wre, F.wim, L.w_stride,
           k0 - K0, F.lt_re, F.lt_im, F.di_re, F.di_im, emit ? sb->l : nullptr,
           emit ? sb->w : nullptr, sb ? sb->stride : 0, sb ? sb->G : 0, sb ? 0 : 1,
-          sb ? sb->l : nullptr, sb ? sb->w : nullptr, ikc_beg, ikc_end, sb ? 1 : 0);
+          sb ? sb->l : nullptr, sb ? sb->w : nullptr, ikc_beg, ikc_end, sb ? 1 : 0, ptiles, K);
       FH_HIP_TRY(hipGetLastError());
       if (sb) continue;
       if (k0 + w < KE)
