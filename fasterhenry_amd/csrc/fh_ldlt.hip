@@ -226,7 +226,7 @@ __device__ __forceinline__ void left_strip_update(float *__restrict__ Re, float 
 template <typename T>
 __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
                             const double *__restrict__ omegas, int K, int N, int64_t ld,
-                            T *__restrict__ re, T *__restrict__ im, int64_t mat_stride) {
+                            T *__restrict__ re, T *__restrict__ im, int64_t mat_stride, int cend) {
   constexpr int V = 16 / static_cast<int>(sizeof(T));
   typedef T TV __attribute__((ext_vector_type(V)));
   const int y = blockIdx.y;
@@ -237,6 +237,7 @@ __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
   // the last row of an odd N pairs its column with itself: once
   if (!first && c2 <= y) return;
   const int c = first ? y : c2;
+  if (c >= cend) return;  // columns the trailing update forms itself
   const int chunk = first ? y / V + i : c2 / V + (i - n1);
   if (chunk >= nch) return;
   const int r0 = chunk * V;
@@ -273,12 +274,12 @@ __global__ void form_kernel(const double *__restrict__ lmesh, int64_t loops,
 template <typename T>
 __global__ void scatter_r_kernel(const int32_t *__restrict__ rows, const int32_t *__restrict__ cols,
                                  const double *__restrict__ vals, int64_t nnz, int64_t ld,
-                                 T *__restrict__ re, int64_t mat_stride) {
+                                 T *__restrict__ re, int64_t mat_stride, int cend) {
   T *Re = re + blockIdx.y * mat_stride;
   for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < nnz;
        e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int32_t r = rows[e], c = cols[e];
-    if (r >= c) Re[static_cast<int64_t>(c) * ld + r] = static_cast<T>(vals[e]);
+    if (r >= c && c < cend) Re[static_cast<int64_t>(c) * ld + r] = static_cast<T>(vals[e]);
   }
 }
 
@@ -980,7 +981,10 @@ __global__ void __launch_bounds__(256) update_left_kernel(
     float *__restrict__ re, float *__restrict__ im, int64_t ld, int64_t mat_stride, int J,
     int nbj, int N, int G, int rtiles, int ctiles, int kc_beg, int kc_end,
     const bf16x8 *__restrict__ lsplit, const bf16x8 *__restrict__ wsplit, int64_t split_stride,
-    int narrow, int K) {
+    int narrow, int K, const double *__restrict__ lmesh, int64_t loops,
+    const double *__restrict__ omegas, const int64_t *__restrict__ rlc_ptr,
+    const int32_t *__restrict__ rlc_row, const double *__restrict__ rlc_val, int r_band,
+    int m_int) {
   int f, tr, tc;
   if (!left_unit(blockIdx.x, rtiles, ctiles, K, f, tr, tc)) return;
   // The wave index is uniform across the wave: keeping it (and every
@@ -1085,15 +1089,50 @@ __global__ void __launch_bounds__(256) update_left_kernel(
 #pragma unroll
   for (int ib = 0; ib < 4; ++ib) {
     float cr[4][4], ci[4][4];
+    if (lmesh) {
+      // This launch is the only one that touches these entries: it forms
+      // R + j w L itself, the values the form and scatter kernels would
+      // have stored (j w L rounded to FP32, the FP32 of the r_mesh entry
+      // or 0), instead of reading them back from the matrix.
+      const double w = omegas[f];
 #pragma unroll
-    for (int jb = 0; jb < 4; ++jb)
+      for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t s = min(s0 + 16 * ib + 4 * kq + v, N - 1);
+          const int64_t r = min(r0 + 16 * jb + x16, N - 1);
+          cr[jb][v] = 0.f;
+          ci[jb][v] = static_cast<float>(w * lmesh[s * loops + r]);
+        }
+      // r_mesh entries lie within r_band below the diagonal in the interior
+      // rows, anywhere in the port rows: most of the wave's 64 x 16
+      // sub-blocks hold none
+      const bool any_r = r0 - (s0 + 16 * ib + 15) <= r_band || r0 + 64 > m_int;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t s = min(s0 + 16 * ib + 4 * kq + v, N - 1);
-        const int64_t r = min(r0 + 16 * jb + x16, N - 1);
-        cr[jb][v] = re[f * mat_stride + s * ld + r];
-        ci[jb][v] = im[f * mat_stride + s * ld + r];
+        const int s = s0 + 16 * ib + 4 * kq + v;
+        if (!any_r || s >= N) continue;
+        for (int64_t e = rlc_ptr[s]; e < rlc_ptr[s + 1]; ++e) {
+          const int d = rlc_row[e] - r0 - x16;  // 16 jb for this lane's rows
+          if (d >= 0 && d < 64 && (d & 15) == 0) {
+            const float rv = static_cast<float>(rlc_val[e]);
+#pragma unroll
+            for (int jb = 0; jb < 4; ++jb)
+              if (d == 16 * jb) cr[jb][v] = rv;
+          }
+        }
       }
+    } else {
+#pragma unroll
+      for (int jb = 0; jb < 4; ++jb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t s = min(s0 + 16 * ib + 4 * kq + v, N - 1);
+          const int64_t r = min(r0 + 16 * jb + x16, N - 1);
+          cr[jb][v] = re[f * mat_stride + s * ld + r];
+          ci[jb][v] = im[f * mat_stride + s * ld + r];
+        }
+    }
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
@@ -1839,9 +1878,12 @@ int64_t split_blocks(const Layout &L) {
   return split_block(KC, 2 * KC, G);
 }
 
+// With `form` (its r_mesh columns and l_mesh) the launch also forms the
+// entries it updates, which no earlier kernel has touched.
 fh_status launch_left_update(const Layout &L, const Factor<float> &F, const SplitBufs &sb, int K,
                              int J, int nbj, int kc_beg, int kc_end, hipStream_t stream,
-                             Profiler *prof) {
+                             Profiler *prof, const DenseSolveInputs *form,
+                             const double *d_omegas) {
   // the kernel takes k in pairs of 32-column chunks
   if (kc_end <= kc_beg || (kc_end - kc_beg) % 2 != 0)
     return set_error(FH_E_INVALID_ARGUMENT, "left-looking update: odd chunk range");
@@ -1850,13 +1892,15 @@ fh_status launch_left_update(const Layout &L, const Factor<float> &F, const Spli
   const int rtiles = narrow ? (L.N - J + 255) / 256 : (L.N - J + 127) / 128;
   const int ctiles = narrow ? 1 : (nbj + 127) / 128;
   const int T = ctiles * (ctiles - 1) / 2 + (rtiles - (ctiles - 1)) * ctiles;
-  if (prof) prof->begin(stream);
   const int64_t units = static_cast<int64_t>(T) * K;
   if (units > 0x7fffff00)
     return set_error(FH_E_INVALID_ARGUMENT, "left-looking update: too many tiles for one launch");
+  if (prof) prof->begin(stream);
   update_left_kernel<<<static_cast<unsigned>((units + 7) / 8 * 8), 256, 0, stream>>>(
       F.re, F.im, L.ld, L.mat_stride, J, nbj, L.N, sb.G, rtiles, ctiles, kc_beg, kc_end, sb.l,
-      sb.w, sb.stride, narrow, K);
+      sb.w, sb.stride, narrow, K, form ? form->d_lmesh : nullptr, static_cast<int64_t>(L.N),
+      d_omegas, form ? form->d_rlc_ptr : nullptr, form ? form->d_rlc_row : nullptr,
+      form ? form->d_rlc_val : nullptr, form ? form->r_band : 0, L.m);
   if (prof) {
     double entries = 0.0;
     for (int s = J; s < J + nbj; ++s) entries += static_cast<double>(L.N - s);
@@ -1874,19 +1918,26 @@ template <typename T>
 fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d_omegas, int K,
                     const Factor<T> &F, int *d_singular, hipStream_t stream, Profiler *prof,
                     const SplitBufs *sb = nullptr) {
+  // On the left-looking path every column right of the first outer block
+  // is first touched by its trailing update, which forms it; only the first
+  // block is formed here.
+  const bool update_forms = std::is_same<T, float>::value && sb && in.d_rlc_ptr && L.m > L.nb;
+  const int cend = update_forms ? L.nb : L.N;
   {
-    // chunks of 16 bytes: a grid row holds at most N / V + 4 of them
+    // chunks of 16 bytes: a grid row holds at most N / V + 4 of them; the
+    // columns c < cend are the grid rows y = c, or y = N - 1 - c when
+    // cend > (N + 1) / 2
     constexpr int V = 16 / static_cast<int>(sizeof(T));
-    dim3 grid(((L.N + V - 1) / V + 4 + 255) / 256, (L.N + 1) / 2);
+    dim3 grid(((L.N + V - 1) / V + 4 + 255) / 256, std::min((L.N + 1) / 2, cend));
     form_kernel<T><<<grid, 256, 0, stream>>>(in.d_lmesh, static_cast<int64_t>(L.N), d_omegas, K,
-                                             L.N, L.ld, F.re, F.im, L.mat_stride);
+                                             L.N, L.ld, F.re, F.im, L.mat_stride, cend);
     FH_HIP_TRY(hipGetLastError());
     if (in.r_nnz) {
       const size_t blocks = std::min<size_t>((in.r_nnz + 255) / 256, 4096);
       dim3 g2(static_cast<unsigned>(blocks), K);
       scatter_r_kernel<T><<<g2, 256, 0, stream>>>(in.d_r_rows, in.d_r_cols, in.d_r_vals,
                                                   static_cast<int64_t>(in.r_nnz), L.ld, F.re,
-                                                  L.mat_stride);
+                                                  L.mat_stride, cend);
       FH_HIP_TRY(hipGetLastError());
     }
   }
@@ -1894,7 +1945,8 @@ fh_status factorize(const DenseSolveInputs &in, const Layout &L, const double *d
     const int KE = std::min(K0 + L.nb, L.m);
     if constexpr (std::is_same<T, float>::value) {
       if (sb && K0 > 0) {
-        const fh_status st = launch_left_update(L, F, *sb, K, K0, KE - K0, 0, K0 / 32, stream, prof);
+        const fh_status st = launch_left_update(L, F, *sb, K, K0, KE - K0, 0, K0 / 32, stream, prof,
+                                                update_forms ? &in : nullptr, d_omegas);
         if (st) return st;
       }
     }

@@ -23,6 +23,15 @@ struct DenseSolveInputs {
   const double *d_ree_val = nullptr;
   const double *d_rep = nullptr;
   const double *d_rpp = nullptr;  // r_pp dense p x p
+  // r_mesh's lower triangle (row >= column) by column over all m + p rows,
+  // rows ascending; may be null (then the matrices are formed in full
+  // before the factorization)
+  const int64_t *d_rlc_ptr = nullptr;
+  const int32_t *d_rlc_row = nullptr;
+  const double *d_rlc_val = nullptr;
+  // the largest row - column of those entries over the m interior rows (the
+  // port rows may hold entries in any column)
+  int r_band = 0;
   // refined form only, may be null: receives x = Zee^-1 Zep of every
   // frequency, 2 K p vectors of m (vector 2 (k p + q) the real part of
   // column q at frequency k, the next one its imaginary part)

@@ -128,6 +128,12 @@ struct fh_system {
   DeviceArray<int64_t> d_ree_ptr;
   DeviceArray<int32_t> d_ree_col;
   DeviceArray<double> d_ree_val, d_rep, d_rpp;
+  // the lower-triangle entries of r_mesh (row >= column) by column, all
+  // N = m + p rows, for forming R + jwL inside the trailing update
+  DeviceArray<int64_t> d_rlc_ptr;
+  DeviceArray<int32_t> d_rlc_row;
+  DeviceArray<double> d_rlc_val;
+  int32_t r_band = 0;  // largest row - column of those entries over the m interior rows
   LdltWorkspace ws;
   // more streams and workspaces: a refined sweep chunk is solved as a few
   // parts side by side, so that one part's latency-bound kernels (the
@@ -340,6 +346,10 @@ fh_status refined_x(fh_system *s, const double *freqs, size_t K, std::vector<dou
   in.d_ree_val = s->d_ree_val.ptr;
   in.d_rep = s->d_rep.ptr;
   in.d_rpp = s->d_rpp.ptr;
+  in.d_rlc_ptr = s->d_rlc_ptr.ptr;
+  in.d_rlc_row = s->d_rlc_row.ptr;
+  in.d_rlc_val = s->d_rlc_val.ptr;
+  in.r_band = s->r_band;
   in.d_x_out = d_x.ptr;
   in.tight_x = true;
   in.refine_stages = s->refine_stages & 15u;
@@ -518,6 +528,39 @@ fh_status fh_system_create_coupled(const fh_problem *problem, fh_solver_choice c
     if (!rpp.empty())
       FH_HIP_TRY(hipMemcpyAsync(s->d_rpp.ptr, rpp.data(), rpp.size() * sizeof(double),
                                 hipMemcpyHostToDevice, s->stream));
+    // r_mesh's lower triangle by column (rows ascending): the entries the
+    // scatter writes (row >= column), for the update that forms them
+    std::vector<std::vector<std::pair<int32_t, double>>> cols(loops);
+    for (size_t e = 0; e < nnz; ++e)
+      if (s->r_rows[e] >= s->r_cols[e])
+        cols[static_cast<size_t>(s->r_cols[e])].emplace_back(s->r_rows[e], s->r_vals[e]);
+    std::vector<int64_t> cptr(loops + 1, 0);
+    std::vector<int32_t> crow;
+    std::vector<double> cval;
+    s->r_band = 0;
+    for (size_t e = 0; e < nnz; ++e)
+      if (s->r_rows[e] >= s->r_cols[e] && static_cast<size_t>(s->r_rows[e]) < m)
+        s->r_band = std::max(s->r_band, s->r_rows[e] - s->r_cols[e]);
+    for (size_t c = 0; c < loops; ++c) {
+      std::sort(cols[c].begin(), cols[c].end());
+      for (auto &e : cols[c]) {
+        crow.push_back(e.first);
+        cval.push_back(e.second);
+      }
+      cptr[c + 1] = static_cast<int64_t>(crow.size());
+    }
+    if ((st = s->d_rlc_ptr.allocate(loops + 1)) ||
+        (st = s->d_rlc_row.allocate(std::max<size_t>(crow.size(), 1))) ||
+        (st = s->d_rlc_val.allocate(std::max<size_t>(cval.size(), 1))))
+      return st;
+    FH_HIP_TRY(hipMemcpyAsync(s->d_rlc_ptr.ptr, cptr.data(), cptr.size() * sizeof(int64_t),
+                              hipMemcpyHostToDevice, s->stream));
+    if (!crow.empty()) {
+      FH_HIP_TRY(hipMemcpyAsync(s->d_rlc_row.ptr, crow.data(), crow.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s->stream));
+      FH_HIP_TRY(hipMemcpyAsync(s->d_rlc_val.ptr, cval.data(), cval.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s->stream));
+    }
     FH_HIP_TRY(hipStreamSynchronize(s->stream));
   }
   s->counts[0] = problem->n_nodes;
@@ -776,6 +819,10 @@ fh_status fh_sweep(fh_system *s, const double *freqs, size_t K, double *z_out,
     in.d_ree_val = s->d_ree_val.ptr;
     in.d_rep = s->d_rep.ptr;
     in.d_rpp = s->d_rpp.ptr;
+    in.d_rlc_ptr = s->d_rlc_ptr.ptr;
+    in.d_rlc_row = s->d_rlc_row.ptr;
+    in.d_rlc_val = s->d_rlc_val.ptr;
+    in.r_band = s->r_band;
     in.refine_stages = s->refine_stages & 15u;
     in.nb = s->nb_for(refined);
     std::vector<double> omegas(chunk);
