@@ -1382,16 +1382,29 @@ __global__ void __launch_bounds__(256) bwd_update_kernel(
   xi[tid] = tid < bw ? oi[B0 + tid] : 0.0;
   const int i0 = blockIdx.x * kB;
   double ar = 0.0, ai = 0.0;
+  // 16-byte loads: V consecutive rows B0 + c of column i per lane, 64 / V
+  // lanes per column, V * 64 / 64 columns i per wave instruction; entries
+  // past bw are zeros (the rows below the super-block are not read)
+  constexpr int V = 16 / static_cast<int>(sizeof(T));
+  typedef T TV __attribute__((ext_vector_type(V)));
+  constexpr int LPC = kB / V;  // lanes per column i
+  const int cq_last = ((bw - 1) / V) * V;
   for (int c0 = 0; c0 < bw; c0 += kB) {
     __syncthreads();
-    const int c = min(c0 + lane, bw - 1);
-#pragma unroll 4
-    for (int k = 0; k < 16; ++k) {
-      const int ii = 16 * wave + k;
+    const int cl = V * (lane % LPC);
+    const int cq = c0 + cl;
+#pragma unroll
+    for (int k = 0; k < 16 * LPC / 64; ++k) {
+      const int ii = 16 * wave + (64 / LPC) * k + lane / LPC;
       const int i = min(i0 + ii, B0 - 1);
-      const int64_t a = static_cast<int64_t>(i) * ld + B0 + c;
-      tr_[ii][lane] = Re[a];
-      ti_[ii][lane] = Im[a];
+      const int64_t a = static_cast<int64_t>(i) * ld + B0 + min(cq, cq_last);
+      const TV vr = *reinterpret_cast<const TV *>(Re + a), vi = *reinterpret_cast<const TV *>(Im + a);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const bool ok = cq + e < bw;
+        tr_[ii][cl + e] = ok ? vr[e] : T(0);
+        ti_[ii][cl + e] = ok ? vi[e] : T(0);
+      }
     }
     __syncthreads();
     // thread (i = lane, quarter = wave): 16 columns of this tile
