@@ -48,7 +48,12 @@ constexpr int kCornerOrder = 16;     // aligned.rs:71
 constexpr double kSnap = 1e-12;      // aligned.rs:74
 constexpr int kTile = 16;
 constexpr int kInlineEvals = 1024;   // point-quadrature evaluations done in the tile
-constexpr int kStackCapacity = 208;  // > the reference's 200-level guard
+// Frames of the aligned quadrature's recursion: the frame at stack index k
+// has depth k and frames of depth 200 are leaves (the reference's guard), so
+// at most 201 are live.  202 frames of 56 bytes plus the wave's 2 KB of
+// lane sums fit twelve one-wave workgroups in a CU's 160 KB of LDS (three
+// per SIMD, the register budget of the kernels that use it).
+constexpr int kStackCapacity = 202;
 
 enum Method : int8_t { kOrthogonal = 0, kBarClosedForm = 1, kAlignedQuadrature = 2,
                        kPointQuadrature = 3, kSampledFilaments = 4, kTruncated = -1 };
@@ -673,8 +678,10 @@ __device__ double corner_wave(const Aligned &b, double axial_start, double a, do
 
 struct Frame {
   double u0, u1, v0, v1, at, acc;
-  int depth, stage, axis, pad;
+  int16_t depth;
+  int8_t stage, axis;
 };
+static_assert(sizeof(Frame) == 56, "Frame layout");
 
 // LDS of a deferred wavefront: the aligned quadrature's stack, or the inner
 // cloud (x, y, z, w) of a heavy point quadrature; one pair uses one of them.
@@ -695,7 +702,7 @@ __device__ __forceinline__ double gap_of(double r0, double r1) {
 __device__ double panel_wave(const Aligned &b, double axial_start, double axial_scale, double u0,
                              double u1, double v0, double v1, Frame *stack, double *lds) {
   int sp = 0;
-  stack[0] = Frame{u0, u1, v0, v1, 0.0, 0.0, 0, 0, 0, 0};
+  stack[0] = Frame{u0, u1, v0, v1, 0.0, 0.0, 0, 0, 0};
   sp = 1;
   double ret = 0.0;
   bool have = false;
