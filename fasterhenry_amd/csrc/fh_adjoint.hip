@@ -43,7 +43,14 @@ constexpr int kCornerOrder = 16;      // aligned.rs:71
 constexpr double kSnap = 1e-12;       // aligned.rs:74
 constexpr int kTile = 16;
 constexpr int kInlineEvals = 1024;
-constexpr int kStackCapacity = 208;
+// The aligned quadrature's stack: at most 201 frames are live (the frame at
+// index k has depth k, and depth 200 is a leaf).  The first kStackLds live in
+// LDS -- with the point cloud they share it with, 16 KB, so that eight
+// one-wave workgroups fit in a CU -- and the rest, reached only by
+// subdivisions 157 deep, in a per-wave area of global memory.
+constexpr int kStackCapacity = 202;
+constexpr int kStackLds = 157;
+constexpr int kStackOvf = kStackCapacity - kStackLds;
 constexpr int kMaxOut = 128;          // K p^2 per contraction pass
 
 __constant__ double a_gx[kMaxRule + 1][kMaxRule];
@@ -636,14 +643,32 @@ __device__ Dual corner_wave(const Aligned &b, Dual axial_start, Dual a, Dual bb,
 
 struct Frame {
   Dual u0, u1, v0, v1, at, acc;
-  int depth, stage, axis, pad;
+  int16_t depth;
+  int8_t stage, axis;
 };
+static_assert(sizeof(Frame) == 104, "Frame layout");
 
-// LDS of a deferred wavefront: the aligned quadrature's stack, or the inner
-// cloud (x, y, z, w) of a heavy point quadrature; one pair uses one of them.
+// LDS of a deferred wavefront: the bottom of the aligned quadrature's stack,
+// or the inner cloud (x, y, z, w) of a heavy point quadrature; one pair uses
+// one of them.
 union DeferredScratch {
-  Frame stack[kStackCapacity];
+  Frame stack[kStackLds];
   Dual cloud[4 * kCloudCapacity];
+};
+static_assert(sizeof(DeferredScratch) == 16384, "16 KB of LDS");
+
+// The wave's stack: frames below kStackLds in LDS, the others in its global
+// overflow area (every lane reads and writes the same values).
+struct FrameStack {
+  Frame *lds, *ovf;
+  __device__ __forceinline__ Frame get(int i) const {
+    if (i < kStackLds) return lds[i];
+    return ovf[i - kStackLds];
+  }
+  __device__ __forceinline__ void put(int i, const Frame &f) const {
+    if (i < kStackLds) lds[i] = f;
+    else ovf[i - kStackLds] = f;
+  }
 };
 
 __device__ __forceinline__ Dual shfl_xor_dual(Dual a, int m) {
@@ -659,15 +684,15 @@ __device__ __forceinline__ double gap_of(double r0, double r1) {
 // aligned.rs:153-199 with the explicit stack of the fill; the subdivision is
 // decided on the values, the panel limits carry their derivatives.
 __device__ Dual panel_wave(const Aligned &b, Dual axial_start, Dual axial_scale, Dual u0, Dual u1, Dual v0,
-                           Dual v1, Frame *stack, Dual *lds) {
+                           Dual v1, const FrameStack &stack, Dual *lds) {
   int sp = 0;
-  stack[0] = Frame{u0, u1, v0, v1, cst(0.0), cst(0.0), 0, 0, 0, 0};
+  stack.put(0, Frame{u0, u1, v0, v1, cst(0.0), cst(0.0), 0, 0, 0});
   sp = 1;
   Dual ret = cst(0.0);
   bool have = false;
   for (;;) {
     if (!have) {
-      Frame f = stack[sp - 1];
+      const Frame f = stack.get(sp - 1);
       const Dual su = f.u1 - f.u0, sv = f.v1 - f.v0;
       int axis = -1;
       Dual at = cst(0.0);
@@ -713,28 +738,31 @@ __device__ Dual panel_wave(const Aligned &b, Dual axial_start, Dual axial_scale,
         have = true;
         --sp;
       } else {
-        stack[sp - 1].stage = 1;
-        stack[sp - 1].axis = axis;
-        stack[sp - 1].at = at;
+        Frame t = f;
+        t.stage = 1;
+        t.axis = static_cast<int8_t>(axis);
+        t.at = at;
+        stack.put(sp - 1, t);
         Frame c = f;
-        c.depth = f.depth + 1;
+        c.depth = static_cast<int16_t>(f.depth + 1);
         c.stage = 0;
         if (axis == 0) c.u1 = at;
         else c.v1 = at;
-        stack[sp++] = c;
+        stack.put(sp++, c);
       }
     } else {
       if (sp == 0) return ret;
-      Frame &p = stack[sp - 1];
+      Frame p = stack.get(sp - 1);
       if (p.stage == 1) {
         p.acc = ret;
         p.stage = 2;
+        stack.put(sp - 1, p);
         Frame c = p;
-        c.depth = p.depth + 1;
+        c.depth = static_cast<int16_t>(p.depth + 1);
         c.stage = 0;
         if (p.axis == 0) c.u0 = p.at;
         else c.v0 = p.at;
-        stack[sp++] = c;
+        stack.put(sp++, c);
         have = false;
       } else {
         ret = p.acc + ret;
@@ -744,7 +772,7 @@ __device__ Dual panel_wave(const Aligned &b, Dual axial_start, Dual axial_scale,
   }
 }
 
-__device__ Dual aligned_quadrature_wave(const Aligned &b, Frame *stack, Dual *lds) {
+__device__ Dual aligned_quadrature_wave(const Aligned &b, const FrameStack &stack, Dual *lds) {
   const Dual axial_start = b.offset[0] + 0.5 * (b.length[0] - b.length[1]);
   const Dual l1 = b.length[0], l2 = b.length[1];
   const Dual sep[4] = {axial_start + l2, axial_start, axial_start + l2 - l1, axial_start - l1};
@@ -765,7 +793,7 @@ __device__ Dual aligned_quadrature_wave(const Aligned &b, Frame *stack, Dual *ld
   return sum;
 }
 
-__device__ Dual aligned_dispatch_wave(const Aligned &b, Frame *stack, Dual *lds) {
+__device__ Dual aligned_dispatch_wave(const Aligned &b, const FrameStack &stack, Dual *lds) {
   const int lane = static_cast<int>(threadIdx.x);
   const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
   const Dual qx = corner(b.length[0], b.length[1], b.offset[0], i);
@@ -788,7 +816,7 @@ __device__ Dual aligned_dispatch_wave(const Aligned &b, Frame *stack, Dual *lds)
 
 // The fill's deferred evaluation (one wave), on duals: 1e-7 x integral.
 __device__ __forceinline__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual *lds, DeferredScratch &sh,
-                                              int lane, bool &ok) {
+                                              Frame *ovf, int lane, bool &ok) {
   Dual integral = cst(0.0);
   ok = true;
   if (kind == kHeavyPoint) {
@@ -828,7 +856,7 @@ __device__ __forceinline__ Dual deferred_eval(const Plan &p, uint32_t kind, Dual
     for (int idx = 0; idx < na; ++idx) total = total + lds[idx];
     integral = total * p.cos;
   } else if (kind == kAlignedKind) {
-    const Dual in = aligned_dispatch_wave(p.ab, sh.stack, lds);
+    const Dual in = aligned_dispatch_wave(p.ab, FrameStack{sh.stack, ovf}, lds);
     const Dual scale = signum(p.cos.v) / (p.area_a * p.area_b);
     integral = in * scale;
   } else {
@@ -1025,7 +1053,7 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
                                                        const unsigned *__restrict__ sidx, unsigned count, int P,
                                                        const unsigned *__restrict__ hsorted,
                                                        const unsigned *__restrict__ unkeyed_begin,
-                                                       double *__restrict__ dl, int *__restrict__ bad) {
+                                                       double *__restrict__ dl, int *__restrict__ bad, Frame *__restrict__ ovf) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
@@ -1045,7 +1073,7 @@ __global__ void __launch_bounds__(64) ddeferred_kernel(const fh_filament *__rest
       Plan p;
       plan_pair(F[f], T[qn + f], F[g], T[qn + g], p);
       bool ok;
-      d = deferred_eval(p, kind, lds, scratch_lds, lane, ok).d;
+      d = deferred_eval(p, kind, lds, scratch_lds, ovf + blockIdx.x * kStackOvf, lane, ok).d;
       if (lane == 0 && !ok) atomicOr(bad, 1);
       if (lane == 0 && p.unsupported) atomicOr(bad, 2);
     }
@@ -1244,7 +1272,7 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
                                                    const unsigned *__restrict__ uq,
                                                    const unsigned *__restrict__ ucount,
                                                    const unsigned *__restrict__ first,
-                                                   double *__restrict__ uval, double *__restrict__ ugrad) {
+                                                   double *__restrict__ uval, double *__restrict__ ugrad, Frame *__restrict__ ovf) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
@@ -1267,12 +1295,12 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
       ab.offset[0] = Dual{c[6], i == 6 ? 1.0 : 0.0};
       ab.offset[1] = Dual{c[7], i == 7 ? 1.0 : 0.0};
       ab.offset[2] = Dual{c[8], i == 8 ? 1.0 : 0.0};
-      v = aligned_dispatch_wave(ab, scratch_lds.stack, lds);
+      v = aligned_dispatch_wave(ab, FrameStack{scratch_lds.stack, ovf + blockIdx.x * kStackOvf}, lds);
     } else {
       Plan p;
       canonical_point_plan(F, qe, aflags[pos], c, i, p);
       bool ok;
-      v = deferred_eval(p, kHeavyPoint, lds, scratch_lds, lane, ok);
+      v = deferred_eval(p, kHeavyPoint, lds, scratch_lds, ovf + blockIdx.x * kStackOvf, lane, ok);
     }
     if (lane == 0) {
       ugrad[w] = v.d;
@@ -1411,7 +1439,7 @@ __global__ void __launch_bounds__(256) dreduce_kernel(const double2 *__restrict_
 __global__ void __launch_bounds__(64) dpair_kernel(const fh_filament *__restrict__ F, int n,
                                                    const fh_filament *__restrict__ T,
                                                    const uint4 *__restrict__ pairs, size_t count,
-                                                   double *__restrict__ value, double *__restrict__ deriv) {
+                                                   double *__restrict__ value, double *__restrict__ deriv, Frame *__restrict__ ovf) {
   __shared__ Dual lds[kCloudCapacity];
   __shared__ DeferredScratch scratch_lds;
   const int lane = static_cast<int>(threadIdx.x);
@@ -1428,7 +1456,7 @@ __global__ void __launch_bounds__(64) dpair_kernel(const fh_filament *__restrict
         v = 1.0e-7 * (point_sum_serial(p.a, p.oa, p.b, p.ob) * p.cos);
       } else {
         const uint32_t kind = p.point ? kHeavyPoint : (p.aligned ? kAlignedKind : kSampled);
-        v = deferred_eval(p, kind, lds, scratch_lds, lane, ok);
+        v = deferred_eval(p, kind, lds, scratch_lds, ovf + blockIdx.x * kStackOvf, lane, ok);
       }
     }
     if (lane == 0) {
@@ -1520,7 +1548,9 @@ fh_status pair_derivatives(const fh_filament *d_F, int n, const fh_filament *d_T
   if (st) return st;
   (void)n;
   const unsigned blocks = static_cast<unsigned>(std::min<size_t>(count, wave_blocks()));
-  dpair_kernel<<<blocks, 64, 0, stream>>>(d_F, n, d_T, d_pairs, count, d_value, d_deriv);
+  DeviceArray<Frame> ovf;  // the stack frames beyond LDS, per wave
+  if ((st = ovf.allocate(static_cast<size_t>(blocks) * kStackOvf))) return st;
+  dpair_kernel<<<blocks, 64, 0, stream>>>(d_F, n, d_T, d_pairs, count, d_value, d_deriv, ovf.ptr);
   FH_HIP_TRY(hipGetLastError());
   FH_HIP_TRY(hipStreamSynchronize(stream));
   return FH_OK;
@@ -1546,6 +1576,7 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
   DeviceArray<uint2> queue;
   DeviceArray<unsigned> qcount;
   DeviceArray<int> flags;  // bit 0: divergent sampled pair, bit 1: unsupported turn
+  DeviceArray<Frame> ovf;  // the aligned quadrature's stack frames beyond LDS, per wave
   const size_t nP = static_cast<size_t>(n) * P;
   if ((st = moving.allocate(nP)) || (st = tpart.allocate(static_cast<size_t>(P) * tiles * nout)) ||
       (st = queue.allocate(pairs)) || (st = qcount.allocate(1)) || (st = res_t.allocate(static_cast<size_t>(P) * nout)) ||
@@ -1626,11 +1657,12 @@ fh_status contract_inductance_derivative(const fh_filament *d_F, int n, const fh
                                         ucount.ptr, nlight.ptr, ubeg.ptr);
     FH_HIP_TRY(hipGetLastError());
     const unsigned dblocks = static_cast<unsigned>(std::min<size_t>(waves, wave_blocks()));
+    if ((st = ovf.allocate(static_cast<size_t>(wave_blocks()) * kStackOvf))) return st;
     ddeferred_kernel<<<dblocks, 64, 0, stream>>>(d_F, n, d_T, moving.ptr, queue.ptr, sidx.ptr, count, P,
-                                                 saidx.ptr, ubeg.ptr, dl.ptr, flags.ptr);
+                                                 saidx.ptr, ubeg.ptr, dl.ptr, flags.ptr, ovf.ptr);
     FH_HIP_TRY(hipGetLastError());
     agrad_kernel<<<wave_blocks(), 64, 0, stream>>>(d_F, queue.ptr, sidx.ptr, akeys.ptr, aflags.ptr, uq.ptr,
-                                                   ucount.ptr, nlight.ptr, uval.ptr, ugrad.ptr);
+                                                   ucount.ptr, nlight.ptr, uval.ptr, ugrad.ptr, ovf.ptr);
     FH_HIP_TRY(hipGetLastError());
     agrad_light_kernel<<<wave_blocks(), 256, 0, stream>>>(d_F, queue.ptr, sidx.ptr, akeys.ptr, aflags.ptr,
                                                           uq.ptr, nlight.ptr, uval.ptr, ugrad.ptr);
