@@ -1168,8 +1168,18 @@ __global__ void akey_kernel(const fh_filament *__restrict__ F, const uint2 *__re
       h ^= h >> 31;
     }
     // the kind in the top two bits: the runs of each kind are contiguous in
-    // hash order, light point quadratures first (kNoAKey sorts last)
-    h = (static_cast<unsigned long long>(kind) << 62) | (h >> 2);
+    // hash order, light point quadratures first (kNoAKey sorts last).  A
+    // light point quadrature's point count (at most kInlineEvals) comes
+    // next, so that the threads of a wave in agrad_light_kernel run loops
+    // of similar lengths.
+    if (kind == kLightPoint) {
+      const unsigned long long pts =
+          static_cast<unsigned long long>(p.oa[0] * p.oa[1] * p.oa[2] * (p.ob[0] * p.ob[1] * p.ob[2]));
+      h = (pts << 51) | (h >> 13);
+    } else {
+      h >>= 2;
+    }
+    h |= static_cast<unsigned long long>(kind) << 62;
   }
   hash[e] = h;
   aflags[e] = fl;
