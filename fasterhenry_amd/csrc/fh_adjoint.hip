@@ -1323,7 +1323,9 @@ __global__ void __launch_bounds__(64) agrad_kernel(const fh_filament *__restrict
 // One thread per (run u, canonical number i), u < *nlight (the light point
 // quadratures): 1e-7 x the serial point sum of the canonical plan, as the
 // tile kernels evaluate a light pair.
-__global__ void __launch_bounds__(256) agrad_light_kernel(const fh_filament *__restrict__ F,
+// Three waves per SIMD (168 VGPRs, 42 spilled) rather than the two its 212
+// registers allow: about 1 % faster on the spiral gradient, same bits.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) agrad_light_kernel(const fh_filament *__restrict__ F,
                                                           const uint2 *__restrict__ queue,
                                                           const unsigned *__restrict__ sidx,
                                                           const double *__restrict__ keys,
